@@ -1,0 +1,589 @@
+"""clspmv_mic_amd -- MI355X-native SpMV engine with the API surface of
+clSpMV (bollig/clSpMV_MIC).
+
+The engine is the C-ABI shared library ``libclspmv.so`` (declared in
+``include/clspmv_formats.h``, ``include/clspmv_device.h`` and
+``include/clspmv_cocktail.h``).  This module loads it with ctypes and offers
+thin Python views of the same entry points for tests, benchmarks and
+scripting:
+
+* ``Coo`` and the other ``*Matrix`` ctypes structs are the C layout twins of
+  the reference's ``coo_matrix<int,float>`` and friends
+  (include/matrix_storage.h:15-365);
+* ``coo2csr`` ... ``coo2sbell`` wrap the conversions (COO -> format);
+* ``Context`` / ``DeviceMatrix`` / ``DeviceVector`` run SpMV on the GPU;
+* ``analyze_matrix`` and ``Cocktail`` expose the multi-format partitioner
+  and its runtime (include/analyze.h, include/spmv_cocktail.h).
+
+There is no CPU fallback: every SpMV runs in the HIP library, and importing
+this module fails when the library has not been built.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libclspmv.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()' "
+        "or make -C clspmv_mic_amd)")
+
+lib = C.CDLL(LIB_PATH)
+
+OK = 0
+ERR_INVALID, ERR_HIP, ERR_ALLOC, ERR_UNSUPPORTED, ERR_TOO_LARGE, ERR_IO = -1, -2, -3, -4, -5, -6
+GPU_ALIGNMENT = 32
+WARPSIZE = 32
+SELL_GROUP_SIZE = 128
+METHOD_AUTO = -1
+
+FMT_CSR, FMT_COO, FMT_ELL, FMT_SELL, FMT_DIA, FMT_BDIA, FMT_BCSR, FMT_BELL, FMT_SBELL = range(9)
+FORMAT_NAMES = ["csr", "coo", "ell", "sell", "dia", "bdia", "bcsr", "bell", "sbell"]
+# Block shapes of the reference's blocked formats (src/analyze.cpp:545-560).
+BLOCK_SHAPES = [(1, 4), (2, 4), (4, 4), (8, 4), (1, 8), (2, 8), (4, 8), (8, 8)]
+
+
+class ClspmvError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        msg = lib.clspmv_last_error().decode(errors="replace")
+        super().__init__(f"{what} failed with status {code}: {msg}")
+        self.code = code
+
+
+def check(code: int, what: str) -> int:
+    if code < 0:
+        raise ClspmvError(code, what)
+    return code
+
+
+# ---------------------------------------------------------------------------
+# structs (include/clspmv_formats.h)
+
+PI = C.POINTER(C.c_int)
+PF = C.POINTER(C.c_float)
+
+
+class MatInfo(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("nnz", C.c_int)]
+
+
+class CooMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("coo_row_id", PI), ("coo_col_id", PI), ("coo_data", PF)]
+
+
+class CsrMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("csr_row_ptr", PI), ("csr_col_id", PI), ("csr_data", PF)]
+
+
+class EllMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("ell_num", C.c_int), ("ell_height_aligned", C.c_int),
+                ("ell_col_id", PI), ("ell_data", PF)]
+
+
+class SellMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("sell_slice_height", C.c_uint), ("sell_slice_num", C.c_int),
+                ("sell_slice_ptr", PI), ("sell_col_id", PI), ("sell_data", PF)]
+
+
+class DiaMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("dia_num", C.c_int), ("dia_length", C.c_int),
+                ("dia_length_aligned", C.c_int), ("dia_offsets", PI), ("dia_data", PF)]
+
+
+DiaExtMatrix = type("DiaExtMatrix", (C.Structure,), {"_fields_": DiaMatrix._fields_})
+
+
+class BdiaMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("bdia_band_num", C.c_int), ("bdia_length", C.c_int),
+                ("bdia_length_aligned", C.c_int), ("bdia_offsets", PI), ("bdia_bptr", PI),
+                ("bdia_data", PF)]
+
+
+class BcsrMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("bcsr_bwidth", C.c_uint), ("bcsr_bheight", C.c_uint),
+                ("bcsr_row_num", C.c_int), ("bcsr_aligned_size", C.c_int), ("bcsr_block_num", C.c_int),
+                ("bcsr_row_ptr", PI), ("bcsr_col_id", PI), ("bcsr_data", PF)]
+
+
+class BellMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("bell_bwidth", C.c_uint), ("bell_bheight", C.c_uint),
+                ("bell_row_num", C.c_int), ("bell_block_num", C.c_int), ("bell_height_aligned", C.c_int),
+                ("bell_float4_aligned", C.c_int), ("bell_col_id", PI), ("bell_data", PF)]
+
+
+class B4csrMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("b4csr_bwidth", C.c_uint), ("b4csr_bheight", C.c_uint),
+                ("b4csr_row_num", C.c_int), ("b4csr_aligned_size", C.c_int), ("b4csr_block_num", C.c_int),
+                ("b4csr_row_ptr", PI), ("b4csr_col_id", PI), ("b4csr_data", PF)]
+
+
+class B4ellMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("b4ell_bwidth", C.c_uint), ("b4ell_bheight", C.c_uint),
+                ("b4ell_row_num", C.c_int), ("b4ell_block_num", C.c_int), ("b4ell_height_aligned", C.c_int),
+                ("b4ell_float4_aligned", C.c_int), ("b4ell_col_id", PI), ("b4ell_data", PF)]
+
+
+class SbellMatrix(C.Structure):
+    _fields_ = [("matinfo", MatInfo), ("sbell_bwidth", C.c_uint), ("sbell_bheight", C.c_uint),
+                ("sbell_slice_height", C.c_uint), ("sbell_slice_num", C.c_int), ("sbell_row_num", C.c_int),
+                ("sbell_slice_ptr", PI), ("sbell_col_id", PI), ("sbell_data", PF)]
+
+
+class CocktailMatrix(C.Structure):
+    _fields_ = ([(f"ifuse{n}", C.c_bool) for n in
+                 ["bdia", "dia", "sbell", "bell", "bcsr", "sell", "ell", "csr", "coo"]] +
+                [(f"{n}_meth_num", C.c_int) for n in
+                 ["bdia", "dia", "sbell", "bell", "bcsr", "sell", "ell", "csr", "coo"]] +
+                [("csr_group_num", C.c_int), ("coo_group_num", C.c_int),
+                 ("mat_width", C.c_int), ("mat_height", C.c_int), ("mat_nnz", C.c_int),
+                 ("bdia", BdiaMatrix), ("dia", DiaMatrix), ("sbell", SbellMatrix),
+                 ("bell", B4ellMatrix), ("bcsr", B4csrMatrix), ("sell", SellMatrix),
+                 ("ell", EllMatrix), ("csr", CsrMatrix), ("coo", CooMatrix)])
+
+
+class DmatInfo(C.Structure):
+    _fields_ = [("format", C.c_int), ("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int),
+                ("stored", C.c_longlong), ("device_bytes", C.c_size_t), ("bheight", C.c_int),
+                ("bwidth", C.c_int), ("slice_height", C.c_int)]
+
+
+class DriverReport(C.Structure):
+    _fields_ = [("format", C.c_int), ("variants", C.c_int), ("best_method", C.c_int),
+                ("best_bheight", C.c_int), ("best_bwidth", C.c_int), ("best_ms", C.c_double),
+                ("best_gflops", C.c_double), ("ms", C.c_double * 64), ("method", C.c_int * 64),
+                ("bheight", C.c_int * 64), ("bwidth", C.c_int * 64), ("max_rel_diff", C.c_double)]
+
+
+class PartTiming(C.Structure):
+    _fields_ = [("format", C.c_int), ("method", C.c_int), ("nnz", C.c_int),
+                ("stored", C.c_longlong), ("ms", C.c_double)]
+
+
+class EvalReport(C.Structure):
+    _fields_ = [("ms_per_spmv", C.c_double), ("gflops", C.c_double), ("l2_distance", C.c_double),
+                ("max_diff", C.c_double), ("max_ratio", C.c_double)]
+
+
+# ---------------------------------------------------------------------------
+# signatures
+
+def _sig(name, restype, *argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = list(argtypes)
+    return f
+
+
+P = C.POINTER
+VP = C.c_void_p
+_sig("clspmv_last_error", C.c_char_p)
+for _n, _T in [("coo", CooMatrix), ("csr", CsrMatrix), ("ell", EllMatrix), ("sell", SellMatrix),
+               ("dia", DiaMatrix), ("dia_ext", DiaExtMatrix), ("bdia", BdiaMatrix),
+               ("bcsr", BcsrMatrix), ("bell", BellMatrix), ("b4csr", B4csrMatrix),
+               ("b4ell", B4ellMatrix), ("sbell", SbellMatrix)]:
+    _sig(f"clspmv_init_{_n}_matrix", None, P(_T))
+    _sig(f"clspmv_free_{_n}_matrix", None, P(_T))
+_sig("clspmv_if_sorted_coo", C.c_int, P(CooMatrix))
+_sig("clspmv_sort_coo", C.c_int, P(CooMatrix))
+_sig("clspmv_aligned_length", C.c_int, C.c_int, C.c_int)
+_sig("clspmv_validate_coo", C.c_int, P(CooMatrix))
+_sig("clspmv_coo2csr", C.c_int, P(CooMatrix), P(CsrMatrix))
+_sig("clspmv_coo2ell", C.c_int, P(CooMatrix), P(EllMatrix), C.c_int, C.c_int)
+_sig("clspmv_coo2sell", C.c_int, P(CooMatrix), P(SellMatrix), C.c_uint)
+_sig("clspmv_coo2dia", C.c_int, P(CooMatrix), P(DiaMatrix), C.c_int)
+_sig("clspmv_coo2diaext", C.c_int, P(CooMatrix), P(DiaExtMatrix), C.c_int)
+_sig("clspmv_coo2bdia", C.c_int, P(CooMatrix), P(BdiaMatrix), C.c_int)
+_sig("clspmv_coo2bcsr", C.c_int, P(CooMatrix), P(BcsrMatrix), C.c_uint, C.c_uint, C.c_int)
+_sig("clspmv_coo2b4csr", C.c_int, P(CooMatrix), P(B4csrMatrix), C.c_uint, C.c_uint, C.c_int)
+_sig("clspmv_coo2bell", C.c_int, P(CooMatrix), P(BellMatrix), C.c_uint, C.c_uint, C.c_int, C.c_int)
+_sig("clspmv_coo2b4ell", C.c_int, P(CooMatrix), P(B4ellMatrix), C.c_uint, C.c_uint, C.c_int, C.c_int)
+_sig("clspmv_coo2sbell", C.c_int, P(CooMatrix), P(SbellMatrix), C.c_uint, C.c_uint, C.c_uint)
+_sig("clspmv_coo2b4csr_nocap", C.c_int, P(CooMatrix), P(B4csrMatrix), C.c_uint, C.c_uint, C.c_int)
+_sig("clspmv_coo2b4ell_nocap", C.c_int, P(CooMatrix), P(B4ellMatrix), C.c_uint, C.c_uint, C.c_int, C.c_int)
+_sig("clspmv_coo2sbell_nocap", C.c_int, P(CooMatrix), P(SbellMatrix), C.c_uint, C.c_uint, C.c_uint)
+for _n, _T in [("csr", CsrMatrix), ("ell", EllMatrix), ("dia", DiaMatrix), ("diaext", DiaExtMatrix),
+               ("bdia", BdiaMatrix), ("bcsr", BcsrMatrix), ("bell", BellMatrix)]:
+    _sig(f"clspmv_{_n}2coo", C.c_int, P(_T), P(CooMatrix))
+_sig("clspmv_read_mmf", C.c_int, C.c_char_p, P(CooMatrix))
+
+_sig("clspmv_ctx_create", C.c_int, C.c_int, P(VP))
+_sig("clspmv_ctx_destroy", C.c_int, VP)
+_sig("clspmv_ctx_synchronize", C.c_int, VP)
+_sig("clspmv_ctx_stream", VP, VP)
+_sig("clspmv_ctx_device", C.c_int, VP)
+_sig("clspmv_device_info", C.c_int, VP, C.c_char_p, C.c_int, P(C.c_int))
+_sig("clspmv_device_malloc", C.c_int, VP, C.c_size_t, P(VP))
+_sig("clspmv_device_free", C.c_int, VP, VP)
+_sig("clspmv_memcpy_h2d", C.c_int, VP, VP, VP, C.c_size_t)
+_sig("clspmv_memcpy_d2h", C.c_int, VP, VP, VP, C.c_size_t)
+_sig("clspmv_memcpy_d2d", C.c_int, VP, VP, VP, C.c_size_t)
+_sig("clspmv_memset_zero", C.c_int, VP, VP, C.c_size_t)
+_sig("clspmv_method_count", C.c_int, C.c_int)
+for _n, _T in [("csr", CsrMatrix), ("coo", CooMatrix), ("ell", EllMatrix), ("sell", SellMatrix),
+               ("dia", DiaMatrix), ("bdia", BdiaMatrix), ("b4csr", B4csrMatrix),
+               ("b4ell", B4ellMatrix), ("sbell", SbellMatrix)]:
+    _sig(f"clspmv_upload_{_n}", C.c_int, VP, P(_T), P(VP))
+_sig("clspmv_dmat_free", C.c_int, VP)
+_sig("clspmv_dmat_get_info", C.c_int, VP, P(DmatInfo))
+_sig("clspmv_spmv", C.c_int, VP, VP, C.c_int, VP, VP)
+_sig("clspmv_time_spmv", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int, C.c_int, P(C.c_double))
+DRIVERS = ["csr_scalar", "csr_vector", "coo", "ell", "sell", "dia", "bdia", "bell", "sbell", "bcsr"]
+for _n in DRIVERS:
+    _sig(f"clspmv_spmv_{_n}", C.c_int, C.c_char_p, P(CooMatrix), C.c_int, C.c_int, C.c_int,
+         P(DriverReport), PF)
+
+if hasattr(lib, "clspmv_analyze_matrix"):
+    _sig("clspmv_init_cocktail", None, P(CocktailMatrix))
+    _sig("clspmv_free_cocktail", None, P(CocktailMatrix))
+    _sig("clspmv_analyze_matrix", C.c_int, P(CooMatrix), P(CocktailMatrix), C.c_float, C.c_int,
+         C.c_int, C.c_char_p)
+if hasattr(lib, "clspmv_init_mat_kernels"):
+    _sig("clspmv_init_mat_kernels", C.c_int, P(CocktailMatrix), PF, PF, P(VP), P(VP), C.c_int)
+    _sig("clspmv_init_mat_kernels_on", C.c_int, C.c_int, P(CocktailMatrix), PF, PF, P(VP), P(VP))
+    _sig("clspmv_do_spmv", C.c_int, VP, C.c_int)
+    _sig("clspmv_do_spmv_async", C.c_int, VP, C.c_int)
+    for _n in ["vector_from_cpu", "result_from_cpu"]:
+        _sig(f"clspmv_cpy_{_n}", C.c_int, VP, VP, PF, C.c_int)
+    for _n in ["vector_from_gpu", "result_from_gpu", "result_to_gpu"]:
+        _sig(f"clspmv_cpy_{_n}", C.c_int, VP, VP, VP, C.c_int)
+    _sig("clspmv_cpy_result_to_cpu", C.c_int, VP, VP, PF, C.c_int)
+    _sig("clspmv_free_mat_kernels", C.c_int, VP, VP)
+    _sig("clspmv_cocktail_gpu_vec", VP, VP)
+    _sig("clspmv_cocktail_gpu_res", VP, VP)
+    _sig("clspmv_cocktail_kernels_ctx", VP, VP)
+    _sig("clspmv_cocktail_profile", C.c_int, VP, C.c_int, P(PartTiming), C.c_int, P(C.c_int),
+         P(C.c_double))
+    _sig("clspmv_evaluate", C.c_int, P(CocktailMatrix), PF, PF, PF, P(EvalReport))
+
+
+# ---------------------------------------------------------------------------
+# host helpers
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(PI)
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(PF)
+
+
+def arr(ptr, n: int, dtype) -> np.ndarray:
+    """Copy n elements behind a ctypes pointer into a numpy array."""
+    if n <= 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Coo:
+    """A COO matrix whose arrays live in numpy; ``.c`` is the C struct
+    pointing at them (valid while this object lives)."""
+
+    def __init__(self, rows, cols, vals, height: int, width: int):
+        self.row = np.ascontiguousarray(rows, dtype=np.int32)
+        self.col = np.ascontiguousarray(cols, dtype=np.int32)
+        self.val = np.ascontiguousarray(vals, dtype=np.float32)
+        assert self.row.shape == self.col.shape == self.val.shape
+        self.height, self.width = int(height), int(width)
+        self.c = CooMatrix()
+        self.c.matinfo = MatInfo(self.width, self.height, int(self.row.size))
+        self.c.coo_row_id = _ip(self.row)
+        self.c.coo_col_id = _ip(self.col)
+        self.c.coo_data = _fp(self.val)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row.size)
+
+    def ref(self):
+        return C.byref(self.c)
+
+    def copy(self) -> "Coo":
+        return Coo(self.row.copy(), self.col.copy(), self.val.copy(), self.height, self.width)
+
+    def sorted(self) -> "Coo":
+        order = np.lexsort((self.col, self.row))
+        return Coo(self.row[order], self.col[order], self.val[order], self.height, self.width)
+
+
+def struct_arrays(m) -> dict:
+    """numpy copies of every array of a format struct, sized from its header."""
+    t = type(m)
+    mi = m.matinfo
+    if t is CooMatrix:
+        n = mi.nnz
+        return dict(row=arr(m.coo_row_id, n, np.int32), col=arr(m.coo_col_id, n, np.int32),
+                    val=arr(m.coo_data, n, np.float32))
+    if t is CsrMatrix:
+        return dict(rp=arr(m.csr_row_ptr, mi.height + 1, np.int32),
+                    col=arr(m.csr_col_id, mi.nnz, np.int32), val=arr(m.csr_data, mi.nnz, np.float32))
+    if t is EllMatrix:
+        n = m.ell_num * m.ell_height_aligned
+        return dict(col=arr(m.ell_col_id, n, np.int32), val=arr(m.ell_data, n, np.float32))
+    if t is SellMatrix:
+        ptr = arr(m.sell_slice_ptr, m.sell_slice_num + 1, np.int32)
+        n = int(ptr[-1]) if ptr.size else 0
+        return dict(ptr=ptr, col=arr(m.sell_col_id, n, np.int32), val=arr(m.sell_data, n, np.float32))
+    if t in (DiaMatrix, DiaExtMatrix):
+        return dict(off=arr(m.dia_offsets, m.dia_num, np.int32),
+                    val=arr(m.dia_data, m.dia_num * m.dia_length_aligned, np.float32))
+    if t is BdiaMatrix:
+        bptr = arr(m.bdia_bptr, m.bdia_band_num + 1, np.int32)
+        nd = int(bptr[-1]) if bptr.size else 0
+        return dict(off=arr(m.bdia_offsets, m.bdia_band_num, np.int32), bptr=bptr,
+                    val=arr(m.bdia_data, nd * m.bdia_length_aligned, np.float32))
+    if t is BcsrMatrix:
+        bs = m.bcsr_bwidth * m.bcsr_bheight
+        return dict(rp=arr(m.bcsr_row_ptr, m.bcsr_row_num + 1, np.int32),
+                    col=arr(m.bcsr_col_id, m.bcsr_block_num, np.int32),
+                    val=arr(m.bcsr_data, m.bcsr_aligned_size * bs, np.float32))
+    if t is BellMatrix:
+        n = m.bell_height_aligned * m.bell_block_num
+        bs = m.bell_bwidth * m.bell_bheight
+        return dict(col=arr(m.bell_col_id, n, np.int32), val=arr(m.bell_data, n * bs, np.float32))
+    if t is B4csrMatrix:
+        return dict(rp=arr(m.b4csr_row_ptr, m.b4csr_row_num + 1, np.int32),
+                    col=arr(m.b4csr_col_id, m.b4csr_block_num, np.int32),
+                    val=arr(m.b4csr_data, m.b4csr_aligned_size * m.b4csr_bheight * (m.b4csr_bwidth // 4),
+                            np.float32))
+    if t is B4ellMatrix:
+        n = m.b4ell_height_aligned * m.b4ell_block_num
+        nd = m.b4ell_float4_aligned * m.b4ell_bheight * (m.b4ell_bwidth // 4) * m.b4ell_block_num
+        return dict(col=arr(m.b4ell_col_id, n, np.int32), val=arr(m.b4ell_data, nd, np.float32))
+    if t is SbellMatrix:
+        ptr = arr(m.sbell_slice_ptr, m.sbell_slice_num + 1, np.int32)
+        n = int(ptr[-1]) if ptr.size else 0
+        return dict(ptr=ptr, col=arr(m.sbell_col_id, n, np.int32),
+                    val=arr(m.sbell_data, n * m.sbell_bwidth * m.sbell_bheight, np.float32))
+    raise TypeError(t)
+
+
+def struct_header(m) -> dict:
+    """Scalar fields of a format struct (no pointers)."""
+    out = {}
+    for name, typ in m._fields_:
+        v = getattr(m, name)
+        if name == "matinfo":
+            out.update(width=v.width, height=v.height, nnz=v.nnz)
+        elif typ in (C.c_int, C.c_uint, C.c_bool):
+            out[name] = int(v)
+    return out
+
+
+_FREE = {CooMatrix: "coo", CsrMatrix: "csr", EllMatrix: "ell", SellMatrix: "sell", DiaMatrix: "dia",
+         DiaExtMatrix: "dia_ext", BdiaMatrix: "bdia", BcsrMatrix: "bcsr", BellMatrix: "bell",
+         B4csrMatrix: "b4csr", B4ellMatrix: "b4ell", SbellMatrix: "sbell"}
+
+
+class Owned:
+    """A format struct filled by the library; frees its arrays on delete."""
+
+    def __init__(self, struct):
+        self.s = struct
+
+    def __del__(self):
+        try:
+            getattr(lib, f"clspmv_free_{_FREE[type(self.s)]}_matrix")(C.byref(self.s))
+        except Exception:
+            pass
+
+    def arrays(self) -> dict:
+        return struct_arrays(self.s)
+
+    def header(self) -> dict:
+        return struct_header(self.s)
+
+
+def _convert(fn: str, T, coo: Coo, *args, expect_bool=False):
+    dst = T()
+    getattr(lib, f"clspmv_init_{_FREE[T]}_matrix")(C.byref(dst))
+    rc = getattr(lib, fn)(coo.ref(), C.byref(dst), *args)
+    check(rc, fn)
+    if expect_bool and rc == 0:
+        return None
+    return Owned(dst)
+
+
+def coo2csr(coo: Coo):
+    return _convert("clspmv_coo2csr", CsrMatrix, coo)
+
+
+def coo2ell(coo: Coo, alignment=GPU_ALIGNMENT, ellnum=0):
+    return _convert("clspmv_coo2ell", EllMatrix, coo, alignment, ellnum)
+
+
+def coo2sell(coo: Coo, slice_height=WARPSIZE):
+    return _convert("clspmv_coo2sell", SellMatrix, coo, slice_height)
+
+
+def coo2dia(coo: Coo, alignment=GPU_ALIGNMENT):
+    return _convert("clspmv_coo2dia", DiaMatrix, coo, alignment, expect_bool=True)
+
+
+def coo2diaext(coo: Coo, alignment=GPU_ALIGNMENT):
+    return _convert("clspmv_coo2diaext", DiaExtMatrix, coo, alignment, expect_bool=True)
+
+
+def coo2bdia(coo: Coo, alignment=GPU_ALIGNMENT):
+    return _convert("clspmv_coo2bdia", BdiaMatrix, coo, alignment, expect_bool=True)
+
+
+def coo2bcsr(coo: Coo, bw: int, bh: int, alignment=GPU_ALIGNMENT):
+    return _convert("clspmv_coo2bcsr", BcsrMatrix, coo, bw, bh, alignment)
+
+
+def coo2b4csr(coo: Coo, bw: int, bh: int, alignment=GPU_ALIGNMENT, cap=True):
+    fn = "clspmv_coo2b4csr" if cap else "clspmv_coo2b4csr_nocap"
+    return _convert(fn, B4csrMatrix, coo, bw, bh, alignment, expect_bool=True)
+
+
+def coo2bell(coo: Coo, bw: int, bh: int, alignment=GPU_ALIGNMENT, ellnum=0):
+    return _convert("clspmv_coo2bell", BellMatrix, coo, bw, bh, alignment, ellnum)
+
+
+def coo2b4ell(coo: Coo, bw: int, bh: int, alignment=GPU_ALIGNMENT, ellnum=0, cap=True):
+    fn = "clspmv_coo2b4ell" if cap else "clspmv_coo2b4ell_nocap"
+    return _convert(fn, B4ellMatrix, coo, bw, bh, alignment, ellnum, expect_bool=True)
+
+
+def coo2sbell(coo: Coo, bw: int, bh: int, slice_height=WARPSIZE, cap=True):
+    fn = "clspmv_coo2sbell" if cap else "clspmv_coo2sbell_nocap"
+    return _convert(fn, SbellMatrix, coo, bw, bh, slice_height, expect_bool=True)
+
+
+def to_coo(owned: Owned) -> Coo:
+    """X -> COO through the library's inverse converter."""
+    name = {CsrMatrix: "csr", EllMatrix: "ell", DiaMatrix: "dia", DiaExtMatrix: "diaext",
+            BdiaMatrix: "bdia", BcsrMatrix: "bcsr", BellMatrix: "bell"}[type(owned.s)]
+    dst = CooMatrix()
+    check(getattr(lib, f"clspmv_{name}2coo")(C.byref(owned.s), C.byref(dst)), f"{name}2coo")
+    o = Owned(dst)
+    a = o.arrays()
+    return Coo(a["row"], a["col"], a["val"], dst.matinfo.height, dst.matinfo.width)
+
+
+def read_mmf(path: str) -> Coo:
+    dst = CooMatrix()
+    check(lib.clspmv_read_mmf(path.encode(), C.byref(dst)), "read_mmf")
+    o = Owned(dst)
+    a = o.arrays()
+    return Coo(a["row"], a["col"], a["val"], dst.matinfo.height, dst.matinfo.width)
+
+
+# ---------------------------------------------------------------------------
+# device side
+
+class Context:
+    """One GPU + one HIP stream (replaces cl_context + cl_command_queue)."""
+
+    def __init__(self, device: int = 0):
+        h = VP()
+        check(lib.clspmv_ctx_create(device, C.byref(h)), "ctx_create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib.clspmv_ctx_destroy(self.h)
+            self.h = VP()
+
+    def __del__(self):
+        self.close()
+
+    def sync(self):
+        check(lib.clspmv_ctx_synchronize(self.h), "ctx_synchronize")
+
+    def info(self):
+        buf = C.create_string_buffer(256)
+        cu = C.c_int()
+        check(lib.clspmv_device_info(self.h, buf, 256, C.byref(cu)), "device_info")
+        return buf.value.decode(), cu.value
+
+
+class DeviceVector:
+    """A float32 device buffer owned through the C-ABI."""
+
+    def __init__(self, ctx: Context, n: int, host: Optional[np.ndarray] = None):
+        self.ctx, self.n = ctx, int(n)
+        p = VP()
+        check(lib.clspmv_device_malloc(ctx.h, max(self.n, 1) * 4, C.byref(p)), "device_malloc")
+        self.p = p
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host: np.ndarray):
+        h = np.ascontiguousarray(host, dtype=np.float32)
+        assert h.size == self.n
+        check(lib.clspmv_memcpy_h2d(self.ctx.h, self.p, h.ctypes.data_as(VP), self.n * 4), "h2d")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float32)
+        check(lib.clspmv_memcpy_d2h(self.ctx.h, out.ctypes.data_as(VP), self.p, self.n * 4), "d2h")
+        return out
+
+    def zero(self):
+        check(lib.clspmv_memset_zero(self.ctx.h, self.p, self.n * 4), "memset")
+
+    def __del__(self):
+        try:
+            if self.p:
+                lib.clspmv_device_free(self.ctx.h, self.p)
+        except Exception:
+            pass
+
+
+_UPLOAD = {CsrMatrix: "csr", CooMatrix: "coo", EllMatrix: "ell", SellMatrix: "sell",
+           DiaMatrix: "dia", BdiaMatrix: "bdia", B4csrMatrix: "b4csr", B4ellMatrix: "b4ell",
+           SbellMatrix: "sbell"}
+
+
+class DeviceMatrix:
+    """A device-resident matrix of any storage format."""
+
+    def __init__(self, ctx: Context, m):
+        s = m.s if isinstance(m, Owned) else (m.c if isinstance(m, Coo) else m)
+        h = VP()
+        check(getattr(lib, f"clspmv_upload_{_UPLOAD[type(s)]}")(ctx.h, C.byref(s), C.byref(h)), "upload")
+        self.ctx, self.h = ctx, h
+        info = DmatInfo()
+        check(lib.clspmv_dmat_get_info(h, C.byref(info)), "dmat_info")
+        self.info = info
+
+    @property
+    def format(self) -> int:
+        return self.info.format
+
+    def methods(self) -> int:
+        return lib.clspmv_method_count(self.info.format)
+
+    def spmv(self, x: DeviceVector, y: DeviceVector, method: int = METHOD_AUTO):
+        """y += A*x (asynchronous on the context's stream)."""
+        check(lib.clspmv_spmv(self.ctx.h, self.h, method, x.p, y.p), "spmv")
+
+    def time(self, x: DeviceVector, y: DeviceVector, method=METHOD_AUTO, warmup=3, ntimes=20) -> float:
+        ms = C.c_double()
+        check(lib.clspmv_time_spmv(self.ctx.h, self.h, method, x.p, y.p, warmup, ntimes, C.byref(ms)),
+              "time_spmv")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.clspmv_dmat_free(self.h)
+        except Exception:
+            pass
+
+
+def run_driver(name: str, coo: Coo, ntimes: int = 10, device: int = 0, want_y: bool = True):
+    """Per-format driver spmv_<name> (include/spmv_<fmt>.h:7)."""
+    rep = DriverReport()
+    y = np.zeros(coo.height, dtype=np.float32) if want_y else None
+    check(getattr(lib, f"clspmv_spmv_{name}")(None, coo.ref(), 1, ntimes, device, C.byref(rep),
+                                               _fp(y) if want_y else None), f"spmv_{name}")
+    return rep, y

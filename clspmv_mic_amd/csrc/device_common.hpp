@@ -1,0 +1,82 @@
+// Device-side helpers shared by the gfx950 SpMV kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clspmv {
+namespace dev {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kBlock = 256;        // work-group size of every row kernel
+constexpr int kXcds = 8;           // MI355X: 8 XCDs, each with its own L2
+
+// Matrix arrays are read exactly once per SpMV: load them non-temporally so
+// they do not evict the x vector from L2 / MALL.
+template <class T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// 128-bit non-temporal loads (the address must be 16-byte aligned).
+__device__ __forceinline__ float4 ld_stream(const float4* p) {
+    const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 ld_stream(const int4* p) {
+    const i32x4 v = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(p));
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
+// Work-groups are dispatched round-robin over the XCDs (group b runs on XCD
+// b % 8).  Remap so that XCD k processes one contiguous range of tiles: rows
+// that share x windows (banded matrices) then share one L2.  Bijective on
+// [0, nb) for any nb.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb / kXcds, r = nb % kXcds;
+    const int x = b % kXcds, i = b / kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// Sum over aligned groups of G consecutive lanes (G a power of two, at most
+// 64); every lane of a group receives the group's total.
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+// Same with a run-time group size (uniform across the wave).
+__device__ __forceinline__ float group_sum_rt(float v, int g) {
+    for (int m = g >> 1; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+// x[e..e+3] as a float4.  The fast path needs e+3 < n and 16-byte alignment
+// of x (checked once on the host, `x_aligned`); otherwise each element is
+// loaded separately and elements at or beyond n read as zero.
+__device__ __forceinline__ float4 ld_x4(const float* __restrict__ x, int e, int n,
+                                        bool x_aligned) {
+    if (x_aligned && e + 3 < n) return *reinterpret_cast<const float4*>(x + e);
+    float4 v;
+    v.x = (e + 0 < n) ? x[e + 0] : 0.0f;
+    v.y = (e + 1 < n) ? x[e + 1] : 0.0f;
+    v.z = (e + 2 < n) ? x[e + 2] : 0.0f;
+    v.w = (e + 3 < n) ? x[e + 3] : 0.0f;
+    return v;
+}
+
+__device__ __forceinline__ float dot4(float4 a, float4 b, float acc) {
+    acc = fmaf(a.x, b.x, acc);
+    acc = fmaf(a.y, b.y, acc);
+    acc = fmaf(a.z, b.z, acc);
+    acc = fmaf(a.w, b.w, acc);
+    return acc;
+}
+
+}  // namespace dev
+}  // namespace clspmv

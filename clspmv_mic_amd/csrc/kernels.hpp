@@ -1,0 +1,60 @@
+// Host-callable launchers of the gfx950 SpMV kernels.  Every launcher
+// enqueues y += A*x on `s` and returns the launch status; shapes are checked
+// by the caller (runtime.cpp) before any launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace clspmv {
+namespace launch {
+
+// ---- CSR ------------------------------------------------------------------
+// Vector kernel: `lanes` (power of two, 1..64) lanes cooperate on one row.
+hipError_t csr_vector(hipStream_t s, int rows, int lanes, const int* rp,
+                      const int* col, const float* val, const float* x, float* y);
+// Adaptive kernel over host-built row blocks (`blocks`, nblocks+1 row starts).
+// A block is either a run of rows whose entries fit kAdaptiveNnz, or one
+// longer row.
+constexpr int kAdaptiveNnz = 1024;
+constexpr int kAdaptiveRows = 256;
+hipError_t csr_adaptive(hipStream_t s, int nblocks, const int* blocks,
+                        const int* rp, const int* col, const float* val,
+                        const float* x, float* y);
+
+// ---- COO ------------------------------------------------------------------
+// Row-sorted COO in chunks of kCooChunk entries; 2 carry slots per chunk.
+constexpr int kCooChunk = 1024;
+hipError_t coo_segreduce(hipStream_t s, int nnz, const int* row, const int* col,
+                         const float* val, const float* x, float* y,
+                         int* carry_row, float* carry_val);
+
+// ---- ELL / SELL --------------------------------------------------------------
+hipError_t ell(hipStream_t s, int rows, int hal, int ellnum, int rows_per_lane,
+               const int* col, const float* val, const float* x, float* y);
+hipError_t sell(hipStream_t s, int rows, int slice_height, const int* ptr,
+                const int* col, const float* val, const float* x, float* y);
+
+// ---- DIA / BDIA -------------------------------------------------------------
+// rows = number of rows computed (min(height, dia_length_aligned)).
+hipError_t dia(hipStream_t s, int rows, int cols, int len, int ndiag,
+               const int* offsets, int min_off, int max_off, int method,
+               const float* data, const float* x, float* y);
+// Largest x window (floats) a work-group stages in LDS.
+constexpr int kLdsWindow = 8192;
+hipError_t bdia(hipStream_t s, int rows, int cols, int len, int nband,
+                const int* band_off, const int* bptr, int max_band_width,
+                int method, const float* data, const float* x, float* y);
+
+// ---- block formats (bh in {1,2,4,8}, bw in {4,8}) ----------------------------
+hipError_t bcsr(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
+                int lanes, int aligned, const int* rp, const int* col4,
+                const float* data, const float* x, bool x_aligned, float* y);
+hipError_t bell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
+                int method, int hal, int f4, int ellnum, const int* col4,
+                const float* data, const float* x, bool x_aligned, float* y);
+hipError_t sbell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
+                 int slice_height, const int* sptr, const int* col4,
+                 const float* data, const float* x, bool x_aligned, float* y);
+
+}  // namespace launch
+}  // namespace clspmv

@@ -1,0 +1,575 @@
+// Device runtime of the clspmv engine: contexts, device-resident matrices of
+// every storage format, launch dispatch, event timing and the per-format
+// benchmark drivers.
+//
+// Uploads replace the reference's cpy_<fmt>_mat (src/spmv_cocktail.cpp:
+// 304-962): the host layouts already give coalesced wave accesses, so they
+// are copied as they are, plus per-format launch metadata computed here
+// (CSR row blocks, DIA offset range, band widths, lane-group sizes).  Column
+// indices are range-checked on upload so that no kernel can read outside x.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <numeric>
+
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+using namespace clspmv;
+
+namespace clspmv {
+
+int use_ctx_device(clspmv_ctx* ctx) {
+    int cur = -1;
+    CLSPMV_HIP(hipGetDevice(&cur));
+    if (cur != ctx->device) CLSPMV_HIP(hipSetDevice(ctx->device));
+    return CLSPMV_OK;
+}
+
+namespace {
+
+int pow2_at_least(int v, int cap) {
+    int p = 1;
+    while (p < v && p < cap) p <<= 1;
+    return p;
+}
+
+int check_cols(const int* c, size_t n, int cols, int scale, const char* what) {
+    for (size_t i = 0; i < n; ++i) {
+        long long e = (long long)c[i] * scale;
+        if (c[i] < 0 || e >= cols)
+            return fail(CLSPMV_ERR_INVALID, "%s: column %lld outside width %d",
+                        what, e, cols);
+    }
+    return CLSPMV_OK;
+}
+
+clspmv_dmat* new_dmat(int format, const clspmv_matinfo& mi) {
+    clspmv_dmat* m = new clspmv_dmat();
+    m->info.format = format;
+    m->info.rows = mi.height;
+    m->info.cols = mi.width;
+    m->info.nnz = mi.nnz;
+    m->info.bheight = 1;
+    m->info.bwidth = 1;
+    hipGetDevice(&m->device);
+    return m;
+}
+
+// Row blocks of the adaptive CSR kernel: a block is a maximal run of at most
+// kAdaptiveRows rows whose entries fit kAdaptiveNnz, or a single longer row.
+std::vector<int> adaptive_blocks(const int* rp, int rows) {
+    std::vector<int> b{0};
+    int r = 0;
+    while (r < rows) {
+        if (rp[r + 1] - rp[r] > launch::kAdaptiveNnz) {
+            b.push_back(++r);
+            continue;
+        }
+        int r2 = r;
+        while (r2 < rows && r2 - r < launch::kAdaptiveRows &&
+               rp[r2 + 1] - rp[r] <= launch::kAdaptiveNnz)
+            ++r2;
+        b.push_back(r2);
+        r = r2;
+    }
+    return b;
+}
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace
+
+int auto_method(const clspmv_dmat* A) {
+    switch (A->info.format) {
+        case CLSPMV_FMT_CSR: {
+            double avg = A->info.rows ? (double)A->info.nnz / A->info.rows : 0.0;
+            return A->max_row > 8.0 * std::max(avg, 4.0) ? 1 : 0;
+        }
+        case CLSPMV_FMT_ELL: return A->hal % 4 == 0 ? 1 : 0;
+        case CLSPMV_FMT_DIA: return A->len % 4 == 0 ? 1 : 0;
+        case CLSPMV_FMT_BDIA:
+            if (4 * 256 + A->max_band <= launch::kLdsWindow / 2) return 4;
+            return A->len % 4 == 0 ? 3 : 0;
+        default: return 0;
+    }
+}
+
+int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
+                const float* x, float* y) {
+    const clspmv_dmat_info& I = A->info;
+    if (method == CLSPMV_METHOD_AUTO) method = auto_method(A);
+    if (method < 0 || method >= clspmv_method_count(I.format))
+        return fail(CLSPMV_ERR_UNSUPPORTED, "method %d not available for format %d",
+                    method, I.format);
+    if (I.rows == 0 || I.cols == 0 || I.nnz == 0) return CLSPMV_OK;
+    hipError_t e = hipSuccess;
+    const bool xa = aligned16(x);
+    switch (I.format) {
+        case CLSPMV_FMT_CSR:
+            e = method == 0 ? launch::csr_vector(s, I.rows, A->lanes, A->rp, A->col, A->val, x, y)
+                            : launch::csr_adaptive(s, A->nblocks, A->blocks, A->rp, A->col, A->val, x, y);
+            break;
+        case CLSPMV_FMT_COO:
+            e = launch::coo_segreduce(s, I.nnz, A->row, A->col, A->val, x, y,
+                                      A->carry_row, A->carry_val);
+            break;
+        case CLSPMV_FMT_ELL:
+            if (method == 1 && A->hal % 4 != 0)
+                return fail(CLSPMV_ERR_UNSUPPORTED, "ELL method 1 needs height_aligned %% 4 == 0");
+            e = launch::ell(s, I.rows, A->hal, A->ellnum, method == 1 ? 4 : 1, A->col, A->val, x, y);
+            break;
+        case CLSPMV_FMT_SELL:
+            e = launch::sell(s, I.rows, I.slice_height, A->sptr, A->col, A->val, x, y);
+            break;
+        case CLSPMV_FMT_DIA: {
+            const int span = A->max_off - A->min_off;
+            if ((method & 1) && A->len % 4)
+                return fail(CLSPMV_ERR_UNSUPPORTED, "DIA method %d needs aligned length %% 4 == 0", method);
+            if (method >= 2 && (method == 3 ? 4 : 1) * 256 + span > launch::kLdsWindow)
+                return fail(CLSPMV_ERR_UNSUPPORTED, "DIA method %d: offset span %d too wide for LDS", method, span);
+            e = launch::dia(s, std::min(I.rows, A->len), I.cols, A->len, A->ndiag, A->offs,
+                            A->min_off, A->max_off, method, A->val, x, y);
+            break;
+        }
+        case CLSPMV_FMT_BDIA: {
+            static const int rpt[5] = {1, 1, 2, 4, 4};
+            static const bool lds[5] = {false, true, true, false, true};
+            if (lds[method] && rpt[method] * 256 + A->max_band > launch::kLdsWindow / 2)
+                return fail(CLSPMV_ERR_UNSUPPORTED, "BDIA method %d: band width %d too wide for LDS", method, A->max_band);
+            if (method == 3 && A->len % 4)
+                return fail(CLSPMV_ERR_UNSUPPORTED, "BDIA method 3 needs aligned length %% 4 == 0");
+            e = launch::bdia(s, std::min(I.rows, A->len), I.cols, A->len, A->nband, A->offs,
+                             A->bptr, A->max_band, method, A->val, x, y);
+            break;
+        }
+        case CLSPMV_FMT_BCSR:
+            e = launch::bcsr(s, A->brows, I.rows, I.cols, I.bheight, I.bwidth, A->lanes,
+                             A->aligned, A->rp, A->col, A->val, x, xa, y);
+            break;
+        case CLSPMV_FMT_BELL:
+            e = launch::bell(s, A->brows, I.rows, I.cols, I.bheight, I.bwidth, method,
+                             A->hal, A->f4, A->ellnum, A->col, A->val, x, xa, y);
+            break;
+        case CLSPMV_FMT_SBELL:
+            e = launch::sbell(s, A->brows, I.rows, I.cols, I.bheight, I.bwidth,
+                              I.slice_height, A->sptr, A->col, A->val, x, xa, y);
+            break;
+        default:
+            return fail(CLSPMV_ERR_INVALID, "unknown format %d", I.format);
+    }
+    if (e != hipSuccess)
+        return fail(CLSPMV_ERR_HIP, "launch failed (format %d method %d): %s", I.format,
+                    method, hipGetErrorString(e));
+    return CLSPMV_OK;
+}
+
+}  // namespace clspmv
+
+extern "C" {
+
+// ---- contexts ----------------------------------------------------------------
+
+int clspmv_ctx_create(int device_ordinal, clspmv_ctx** out) {
+    if (!out) return fail(CLSPMV_ERR_INVALID, "null output");
+    int n = 0;
+    CLSPMV_HIP(hipGetDeviceCount(&n));
+    if (device_ordinal < 0 || device_ordinal >= n)
+        return fail(CLSPMV_ERR_INVALID, "device %d not present (%d devices)", device_ordinal, n);
+    CLSPMV_HIP(hipSetDevice(device_ordinal));
+    clspmv_ctx* c = new clspmv_ctx();
+    c->device = device_ordinal;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_stop);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(CLSPMV_ERR_HIP, "stream/event creation failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return CLSPMV_OK;
+}
+
+int clspmv_ctx_destroy(clspmv_ctx* ctx) {
+    if (!ctx) return CLSPMV_OK;
+    use_ctx_device(ctx);
+    hipStreamSynchronize(ctx->stream);
+    hipEventDestroy(ctx->ev_start);
+    hipEventDestroy(ctx->ev_stop);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CLSPMV_OK;
+}
+
+int clspmv_ctx_synchronize(clspmv_ctx* ctx) {
+    if (!ctx) return fail(CLSPMV_ERR_INVALID, "null context");
+    CLSPMV_HIP(hipStreamSynchronize(ctx->stream));
+    return CLSPMV_OK;
+}
+
+void* clspmv_ctx_stream(clspmv_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+int clspmv_ctx_device(clspmv_ctx* ctx) { return ctx ? ctx->device : -1; }
+
+int clspmv_device_info(clspmv_ctx* ctx, char* name, int name_len, int* cu_count) {
+    if (!ctx) return fail(CLSPMV_ERR_INVALID, "null context");
+    hipDeviceProp_t p;
+    CLSPMV_HIP(hipGetDeviceProperties(&p, ctx->device));
+    if (name && name_len > 0) {
+        std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    return CLSPMV_OK;
+}
+
+int clspmv_device_malloc(clspmv_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(CLSPMV_ERR_INVALID, "null argument");
+    int st = use_ctx_device(ctx);
+    if (st) return st;
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(CLSPMV_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return CLSPMV_OK;
+}
+
+int clspmv_device_free(clspmv_ctx* ctx, void* ptr) {
+    if (!ctx) return fail(CLSPMV_ERR_INVALID, "null context");
+    if (ptr) CLSPMV_HIP(hipFree(ptr));
+    return CLSPMV_OK;
+}
+
+int clspmv_memcpy_h2d(clspmv_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return fail(CLSPMV_ERR_INVALID, "null context");
+    if (!bytes) return CLSPMV_OK;
+    CLSPMV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    CLSPMV_HIP(hipStreamSynchronize(ctx->stream));
+    return CLSPMV_OK;
+}
+
+int clspmv_memcpy_d2h(clspmv_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return fail(CLSPMV_ERR_INVALID, "null context");
+    if (!bytes) return CLSPMV_OK;
+    CLSPMV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    CLSPMV_HIP(hipStreamSynchronize(ctx->stream));
+    return CLSPMV_OK;
+}
+
+int clspmv_memcpy_d2d(clspmv_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return fail(CLSPMV_ERR_INVALID, "null context");
+    if (!bytes) return CLSPMV_OK;
+    CLSPMV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return CLSPMV_OK;
+}
+
+int clspmv_memset_zero(clspmv_ctx* ctx, void* dst, size_t bytes) {
+    if (!ctx) return fail(CLSPMV_ERR_INVALID, "null context");
+    if (!bytes) return CLSPMV_OK;
+    CLSPMV_HIP(hipMemsetAsync(dst, 0, bytes, ctx->stream));
+    return CLSPMV_OK;
+}
+
+// ---- device matrices -----------------------------------------------------------
+
+int clspmv_method_count(int format) {
+    switch (format) {
+        case CLSPMV_FMT_CSR: return CLSPMV_CSR_IMP_NUM;
+        case CLSPMV_FMT_COO: return CLSPMV_COO_IMP_NUM;
+        case CLSPMV_FMT_ELL: return CLSPMV_ELL_IMP_NUM;
+        case CLSPMV_FMT_SELL: return CLSPMV_SELL_IMP_NUM;
+        case CLSPMV_FMT_DIA: return CLSPMV_DIA_IMP_NUM;
+        case CLSPMV_FMT_BDIA: return CLSPMV_BDIA_IMP_NUM;
+        case CLSPMV_FMT_BCSR: return CLSPMV_BCSR_IMP_NUM;
+        case CLSPMV_FMT_BELL: return CLSPMV_BELL_IMP_NUM;
+        case CLSPMV_FMT_SBELL: return CLSPMV_SBELL_IMP_NUM;
+        default: return 0;
+    }
+}
+
+#define UPLOAD_PROLOGUE(m, fmt)                                               \
+    if (!ctx || !m || !out) return fail(CLSPMV_ERR_INVALID, "null argument"); \
+    {                                                                         \
+        int st0 = use_ctx_device(ctx);                                        \
+        if (st0) return st0;                                                  \
+    }                                                                         \
+    clspmv_dmat* d = new_dmat(fmt, m->matinfo);                               \
+    int st = CLSPMV_OK;
+
+#define UPLOAD_CHECK(expr)            \
+    do {                              \
+        st = (expr);                  \
+        if (st) {                     \
+            clspmv_dmat_free(d);      \
+            return st;                \
+        }                             \
+    } while (0)
+
+int clspmv_upload_csr(clspmv_ctx* ctx, const clspmv_csr_matrix* m, clspmv_dmat** out) {
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_CSR)
+    const int rows = m->matinfo.height, nnz = m->matinfo.nnz;
+    if (rows < 0 || nnz < 0 || (rows > 0 && !m->csr_row_ptr))
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed CSR matrix"));
+    if (rows > 0 && (m->csr_row_ptr[0] != 0 || m->csr_row_ptr[rows] != nnz))
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "CSR row pointer does not span nnz"));
+    int max_row = 0;
+    for (int r = 0; r < rows; ++r) {
+        int len = m->csr_row_ptr[r + 1] - m->csr_row_ptr[r];
+        if (len < 0) UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "CSR row pointer decreases at %d", r));
+        max_row = std::max(max_row, len);
+    }
+    UPLOAD_CHECK(check_cols(m->csr_col_id, nnz, m->matinfo.width, 1, "csr"));
+    d->info.stored = nnz;
+    d->max_row = max_row;
+    double avg = rows ? (double)nnz / rows : 0.0;
+    d->lanes = pow2_at_least((int)std::ceil(avg), 64);
+    UPLOAD_CHECK(upload_array(d, m->csr_row_ptr, (size_t)rows + 1, &d->rp));
+    UPLOAD_CHECK(upload_array(d, m->csr_col_id, nnz, &d->col));
+    UPLOAD_CHECK(upload_array(d, m->csr_data, nnz, &d->val));
+    std::vector<int> blocks = adaptive_blocks(m->csr_row_ptr, rows);
+    d->nblocks = (int)blocks.size() - 1;
+    UPLOAD_CHECK(upload_array(d, blocks.data(), blocks.size(), &d->blocks));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+int clspmv_upload_coo(clspmv_ctx* ctx, const clspmv_coo_matrix* m, clspmv_dmat** out) {
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_COO)
+    UPLOAD_CHECK(clspmv_validate_coo(m));
+    const int nnz = m->matinfo.nnz;
+    d->info.stored = nnz;
+    // The segmented reduction needs row-sorted entries: sort a copy if needed.
+    clspmv_coo_matrix sorted = *m;
+    std::vector<int> r, c;
+    std::vector<float> v;
+    if (!coo_sorted(*m)) {
+        r.assign(m->coo_row_id, m->coo_row_id + nnz);
+        c.assign(m->coo_col_id, m->coo_col_id + nnz);
+        v.assign(m->coo_data, m->coo_data + nnz);
+        sorted.coo_row_id = r.data();
+        sorted.coo_col_id = c.data();
+        sorted.coo_data = v.data();
+        coo_sort(sorted);
+    }
+    UPLOAD_CHECK(upload_array(d, sorted.coo_row_id, nnz, &d->row));
+    UPLOAD_CHECK(upload_array(d, sorted.coo_col_id, nnz, &d->col));
+    UPLOAD_CHECK(upload_array(d, sorted.coo_data, nnz, &d->val));
+    const int chunks = (nnz + launch::kCooChunk - 1) / launch::kCooChunk;
+    UPLOAD_CHECK(upload_array<int>(d, nullptr, 2 * (size_t)chunks, &d->carry_row));
+    UPLOAD_CHECK(upload_array<float>(d, nullptr, 2 * (size_t)chunks, &d->carry_val));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+int clspmv_upload_ell(clspmv_ctx* ctx, const clspmv_ell_matrix* m, clspmv_dmat** out) {
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_ELL)
+    const size_t total = (size_t)m->ell_height_aligned * m->ell_num;
+    if (m->ell_height_aligned < m->matinfo.height || m->ell_num < 0)
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed ELL matrix"));
+    UPLOAD_CHECK(check_cols(m->ell_col_id, total, std::max(m->matinfo.width, 1), 1, "ell"));
+    d->hal = m->ell_height_aligned;
+    d->ellnum = m->ell_num;
+    d->info.stored = (long long)total;
+    UPLOAD_CHECK(upload_array(d, m->ell_col_id, total, &d->col));
+    UPLOAD_CHECK(upload_array(d, m->ell_data, total, &d->val));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+int clspmv_upload_sell(clspmv_ctx* ctx, const clspmv_sell_matrix* m, clspmv_dmat** out) {
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_SELL)
+    const int h = (int)m->sell_slice_height, ns = m->sell_slice_num;
+    if (h <= 0 || ns < 0 || (long long)ns * h < m->matinfo.height || !m->sell_slice_ptr)
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed SELL matrix"));
+    for (int s = 0; s < ns; ++s) {
+        int w = m->sell_slice_ptr[s + 1] - m->sell_slice_ptr[s];
+        if (w < 0 || w % h) UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "SELL slice %d size %d", s, w));
+    }
+    const size_t total = (size_t)m->sell_slice_ptr[ns];
+    UPLOAD_CHECK(check_cols(m->sell_col_id, total, std::max(m->matinfo.width, 1), 1, "sell"));
+    d->info.slice_height = h;
+    d->info.stored = (long long)total;
+    UPLOAD_CHECK(upload_array(d, m->sell_slice_ptr, (size_t)ns + 1, &d->sptr));
+    UPLOAD_CHECK(upload_array(d, m->sell_col_id, total, &d->col));
+    UPLOAD_CHECK(upload_array(d, m->sell_data, total, &d->val));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+int clspmv_upload_dia(clspmv_ctx* ctx, const clspmv_dia_matrix* m, clspmv_dmat** out) {
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_DIA)
+    if (m->dia_num < 0 || m->dia_length_aligned < 0 || (m->dia_num > 0 && !m->dia_offsets))
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed DIA matrix"));
+    d->len = m->dia_length_aligned;
+    d->ndiag = m->dia_num;
+    d->min_off = d->max_off = 0;
+    for (int j = 0; j < m->dia_num; ++j) {
+        if (j == 0 || m->dia_offsets[j] < d->min_off) d->min_off = m->dia_offsets[j];
+        if (j == 0 || m->dia_offsets[j] > d->max_off) d->max_off = m->dia_offsets[j];
+    }
+    d->info.stored = (long long)m->dia_num * m->dia_length_aligned;
+    UPLOAD_CHECK(upload_array(d, m->dia_offsets, (size_t)m->dia_num, &d->offs));
+    UPLOAD_CHECK(upload_array(d, m->dia_data, (size_t)m->dia_num * m->dia_length_aligned, &d->val));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+int clspmv_upload_bdia(clspmv_ctx* ctx, const clspmv_bdia_matrix* m, clspmv_dmat** out) {
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_BDIA)
+    const int nb = m->bdia_band_num;
+    if (nb < 0 || m->bdia_length_aligned < 0 || (nb > 0 && (!m->bdia_offsets || !m->bdia_bptr)))
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed BDIA matrix"));
+    d->len = m->bdia_length_aligned;
+    d->nband = nb;
+    d->max_band = 0;
+    for (int b = 0; b < nb; ++b) {
+        int w = m->bdia_bptr[b + 1] - m->bdia_bptr[b];
+        if (w <= 0) UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "BDIA band %d is empty", b));
+        d->max_band = std::max(d->max_band, w);
+    }
+    const int ndiag = nb > 0 ? m->bdia_bptr[nb] : 0;
+    d->ndiag = ndiag;
+    d->info.stored = (long long)ndiag * m->bdia_length_aligned;
+    UPLOAD_CHECK(upload_array(d, m->bdia_offsets, (size_t)nb, &d->offs));
+    UPLOAD_CHECK(upload_array(d, m->bdia_bptr, (size_t)nb + 1, &d->bptr));
+    UPLOAD_CHECK(upload_array(d, m->bdia_data, (size_t)ndiag * m->bdia_length_aligned, &d->val));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+static int block_shape_ok(unsigned bh, unsigned bw) {
+    return (bh == 1 || bh == 2 || bh == 4 || bh == 8) && (bw == 4 || bw == 8);
+}
+
+int clspmv_upload_b4csr(clspmv_ctx* ctx, const clspmv_b4csr_matrix* m, clspmv_dmat** out) {
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_BCSR)
+    const unsigned bh = m->b4csr_bheight, bw = m->b4csr_bwidth;
+    if (!block_shape_ok(bh, bw))
+        UPLOAD_CHECK(fail(CLSPMV_ERR_UNSUPPORTED, "b4csr block %ux%u not supported", bh, bw));
+    const int nbr = m->b4csr_row_num, nb = m->b4csr_block_num;
+    if (nbr < 0 || nb < 0 || (long long)nbr * bh < m->matinfo.height ||
+        m->b4csr_aligned_size < 4LL * nb || !m->b4csr_row_ptr)
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed b4csr matrix"));
+    // col4 + bw/4 - 1 float4s must start inside the padded width.
+    UPLOAD_CHECK(check_cols(m->b4csr_col_id, nb, std::max(m->matinfo.width, 1), 4, "b4csr"));
+    d->info.bheight = (int)bh;
+    d->info.bwidth = (int)bw;
+    d->brows = nbr;
+    d->aligned = m->b4csr_aligned_size;
+    d->info.stored = (long long)nb * bh * bw;
+    double avg = nbr ? (double)nb / nbr : 0.0;
+    d->lanes = pow2_at_least((int)std::ceil(avg), 64);
+    UPLOAD_CHECK(upload_array(d, m->b4csr_row_ptr, (size_t)nbr + 1, &d->rp));
+    UPLOAD_CHECK(upload_array(d, m->b4csr_col_id, (size_t)nb, &d->col));
+    UPLOAD_CHECK(upload_array(d, m->b4csr_data, (size_t)m->b4csr_aligned_size * bh * (bw / 4), &d->val));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+int clspmv_upload_b4ell(clspmv_ctx* ctx, const clspmv_b4ell_matrix* m, clspmv_dmat** out) {
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_BELL)
+    const unsigned bh = m->b4ell_bheight, bw = m->b4ell_bwidth;
+    if (!block_shape_ok(bh, bw))
+        UPLOAD_CHECK(fail(CLSPMV_ERR_UNSUPPORTED, "b4ell block %ux%u not supported", bh, bw));
+    const int nbr = m->b4ell_row_num, ell = m->b4ell_block_num;
+    if (nbr < 0 || ell < 0 || (long long)nbr * bh < m->matinfo.height ||
+        m->b4ell_height_aligned < nbr || m->b4ell_float4_aligned < 4LL * nbr)
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed b4ell matrix"));
+    const size_t ncol = (size_t)m->b4ell_height_aligned * ell;
+    // Check the columns of real block rows only; padding rows are never read.
+    for (int s = 0; s < ell; ++s)
+        UPLOAD_CHECK(check_cols(m->b4ell_col_id + (size_t)s * m->b4ell_height_aligned, nbr,
+                                std::max(m->matinfo.width, 1), 4, "b4ell"));
+    d->info.bheight = (int)bh;
+    d->info.bwidth = (int)bw;
+    d->brows = nbr;
+    d->hal = m->b4ell_height_aligned;
+    d->f4 = m->b4ell_float4_aligned;
+    d->ellnum = ell;
+    d->info.stored = (long long)nbr * ell * bh * bw;
+    UPLOAD_CHECK(upload_array(d, m->b4ell_col_id, ncol, &d->col));
+    UPLOAD_CHECK(upload_array(d, m->b4ell_data, (size_t)m->b4ell_float4_aligned * bh * (bw / 4) * ell, &d->val));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+int clspmv_upload_sbell(clspmv_ctx* ctx, const clspmv_sbell_matrix* m, clspmv_dmat** out) {
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_SBELL)
+    const unsigned bh = m->sbell_bheight, bw = m->sbell_bwidth;
+    if (!block_shape_ok(bh, bw))
+        UPLOAD_CHECK(fail(CLSPMV_ERR_UNSUPPORTED, "sbell block %ux%u not supported", bh, bw));
+    const int nbr = m->sbell_row_num, ns = m->sbell_slice_num, sh = (int)m->sbell_slice_height;
+    if (nbr < 0 || ns < 0 || sh <= 0 || (long long)ns * sh < nbr ||
+        (long long)nbr * bh < m->matinfo.height || !m->sbell_slice_ptr)
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed sbell matrix"));
+    for (int s = 0; s < ns; ++s) {
+        int w = m->sbell_slice_ptr[s + 1] - m->sbell_slice_ptr[s];
+        if (w < 0 || w % sh) UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "sbell slice %d size %d", s, w));
+    }
+    const size_t total = (size_t)m->sbell_slice_ptr[ns];
+    UPLOAD_CHECK(check_cols(m->sbell_col_id, total, std::max(m->matinfo.width, 1), 4, "sbell"));
+    d->info.bheight = (int)bh;
+    d->info.bwidth = (int)bw;
+    d->info.slice_height = sh;
+    d->brows = nbr;
+    d->info.stored = (long long)total * bh * bw;
+    UPLOAD_CHECK(upload_array(d, m->sbell_slice_ptr, (size_t)ns + 1, &d->sptr));
+    UPLOAD_CHECK(upload_array(d, m->sbell_col_id, total, &d->col));
+    UPLOAD_CHECK(upload_array(d, m->sbell_data, total * bh * bw, &d->val));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+#undef UPLOAD_PROLOGUE
+#undef UPLOAD_CHECK
+
+int clspmv_dmat_free(clspmv_dmat* m) {
+    if (!m) return CLSPMV_OK;
+    int cur = -1;
+    hipGetDevice(&cur);
+    if (cur != m->device) hipSetDevice(m->device);
+    for (void* p : m->bufs) hipFree(p);
+    if (cur != m->device && cur >= 0) hipSetDevice(cur);
+    delete m;
+    return CLSPMV_OK;
+}
+
+int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out) {
+    if (!m || !out) return fail(CLSPMV_ERR_INVALID, "null argument");
+    *out = m->info;
+    return CLSPMV_OK;
+}
+
+int clspmv_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
+                const float* d_x, float* d_y) {
+    if (!ctx || !A) return fail(CLSPMV_ERR_INVALID, "null argument");
+    if (A->info.nnz > 0 && A->info.rows > 0 && (!d_x || !d_y))
+        return fail(CLSPMV_ERR_INVALID, "null vector");
+    if (A->device != ctx->device)
+        return fail(CLSPMV_ERR_INVALID, "matrix lives on device %d, context on %d",
+                    A->device, ctx->device);
+    int st = use_ctx_device(ctx);
+    if (st) return st;
+    return dmat_launch(ctx->stream, A, method, d_x, d_y);
+}
+
+int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
+                     const float* d_x, float* d_y, int warmup, int ntimes,
+                     double* ms_per_call) {
+    if (!ctx || !A || ntimes <= 0) return fail(CLSPMV_ERR_INVALID, "bad argument");
+    for (int i = 0; i < warmup; ++i) {
+        int st = clspmv_spmv(ctx, A, method, d_x, d_y);
+        if (st) return st;
+    }
+    CLSPMV_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    for (int i = 0; i < ntimes; ++i) {
+        int st = clspmv_spmv(ctx, A, method, d_x, d_y);
+        if (st) return st;
+    }
+    CLSPMV_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    CLSPMV_HIP(hipEventSynchronize(ctx->ev_stop));
+    float ms = 0.0f;
+    CLSPMV_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (ms_per_call) *ms_per_call = (double)ms / ntimes;
+    return CLSPMV_OK;
+}
+
+}  // extern "C"

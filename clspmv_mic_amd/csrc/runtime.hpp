@@ -1,0 +1,87 @@
+// Internal definitions of the device runtime shared by runtime.cpp (contexts,
+// per-format matrices, drivers) and cocktail.cpp (the Cocktail runtime).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "clspmv_device.h"
+#include "common.hpp"
+
+struct clspmv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+};
+
+struct clspmv_dmat {
+    clspmv_dmat_info info{};
+    int device = 0;
+    std::vector<void*> bufs;  // device allocations owned by the matrix
+
+    // CSR: rp/col/val; adaptive row blocks; vector lanes per row.
+    int* rp = nullptr;
+    int* col = nullptr;
+    float* val = nullptr;
+    int* blocks = nullptr;
+    int nblocks = 0;
+    int lanes = 1;
+    int max_row = 0;
+    // COO: row/col/val, carry slots (2 per chunk).
+    int* row = nullptr;
+    int* carry_row = nullptr;
+    float* carry_val = nullptr;
+    // ELL: col/val, height_aligned, ell_num.  SELL: sptr/col/val.
+    int hal = 0, ellnum = 0;
+    int* sptr = nullptr;
+    // DIA / BDIA: offsets, band pointers, aligned length.
+    int* offs = nullptr;
+    int* bptr = nullptr;
+    int len = 0, ndiag = 0, nband = 0, min_off = 0, max_off = 0, max_band = 0;
+    // Block formats: block rows, aligned sizes, group lanes.
+    int brows = 0, aligned = 0, f4 = 0;
+};
+
+namespace clspmv {
+
+// HIP call wrapper: records the failing call and returns CLSPMV_ERR_HIP.
+#define CLSPMV_HIP(call)                                                        \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess)                                                   \
+            return ::clspmv::fail(CLSPMV_ERR_HIP, "%s failed: %s (%s:%d)", #call,  \
+                                  hipGetErrorString(e_), __FILE__, __LINE__);   \
+    } while (0)
+
+// Device copy of a host array, owned by `m`.
+template <class T>
+int upload_array(clspmv_dmat* m, const T* host, size_t n, T** out) {
+    void* p = nullptr;
+    size_t bytes = sizeof(T) * (n ? n : 1);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+        return fail(CLSPMV_ERR_ALLOC, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    m->bufs.push_back(p);
+    m->info.device_bytes += bytes;
+    if (n && host) {
+        e = hipMemcpy(p, host, sizeof(T) * n, hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+            return fail(CLSPMV_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(e));
+    }
+    *out = static_cast<T*>(p);
+    return CLSPMV_OK;
+}
+
+// Checks that the context's device is current; sets it otherwise.
+int use_ctx_device(clspmv_ctx* ctx);
+
+// Launch y += A*x for one device matrix on `stream` (no argument checks
+// beyond the method's shape requirements).
+int dmat_launch(hipStream_t stream, const clspmv_dmat* A, int method,
+                const float* x, float* y);
+
+// Method chosen for CLSPMV_METHOD_AUTO.
+int auto_method(const clspmv_dmat* A);
+
+}  // namespace clspmv

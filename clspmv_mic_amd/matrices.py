@@ -1,0 +1,178 @@
+"""Synthetic matrices of the engine's benchmark and test workloads.
+
+Every generator is deterministic in its seed (numpy PCG64) and returns a
+row-sorted COO without duplicate entries (``clspmv_mic_amd.Coo``).
+
+* ``laplacian_2d(n)``       -- 5-point Laplacian on an n x n grid, diagonal
+                               4.0, off-diagonals -1.0.
+* ``stencil_27pt(n)``       -- 27-point stencil on an n^3 grid, N(0,1)
+                               off-diagonals, diagonally dominant centre
+                               (|centre| = sum |off-diagonals| + 1).
+* ``power_law(m)``          -- row lengths Zipf(1.8) clipped to [1, 5000],
+                               uniform random columns, N(0,1) values, united
+                               with a band at offsets {0, +-1, +-1000}.
+* ``block_stencil(nbr)``    -- 4x4 dense blocks, 12 blocks per block row at
+                               block-column offsets {0, +-1, +-32, +-33,
+                               +-4096, +-4097, +31} (mod nbr), N(0,1).
+* ``random_sparse(...)``    -- small random matrices for parity tests,
+                               optionally with dyadic values (exact FP32
+                               sums) and empty / long rows.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import Coo
+
+
+def _finish(rows, cols, vals, height, width) -> Coo:
+    key = rows.astype(np.int64) * width + cols.astype(np.int64)
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    keep = np.ones(key.size, dtype=bool)
+    keep[1:] = key[1:] != key[:-1]          # first occurrence wins
+    order = order[keep]
+    return Coo(rows[order], cols[order], vals[order], height, width)
+
+
+def laplacian_2d(n: int = 256) -> Coo:
+    m = n * n
+    r = np.arange(m, dtype=np.int64)
+    i, j = r // n, r % n
+    parts = [(r, r, np.full(m, 4.0, np.float32))]
+    for di, dj in [(-1, 0), (1, 0), (0, -1), (0, 1)]:
+        ok = (i + di >= 0) & (i + di < n) & (j + dj >= 0) & (j + dj < n)
+        parts.append((r[ok], (r + di * n + dj)[ok], np.full(int(ok.sum()), -1.0, np.float32)))
+    rows = np.concatenate([p[0] for p in parts])
+    cols = np.concatenate([p[1] for p in parts])
+    vals = np.concatenate([p[2] for p in parts])
+    return _finish(rows.astype(np.int32), cols.astype(np.int32), vals, m, m)
+
+
+def stencil_27pt(n: int = 128, seed: int = 3, dyadic: bool = False) -> Coo:
+    """27-point stencil; rows in natural (x fastest) order."""
+    m = n ** 3
+    rng = np.random.default_rng(seed)
+    r = np.arange(m, dtype=np.int64)
+    x, y, z = r % n, (r // n) % n, r // (n * n)
+    rows, cols, vals = [], [], []
+    absum = np.zeros(m, np.float64)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if dx == dy == dz == 0:
+                    continue
+                ok = ((x + dx >= 0) & (x + dx < n) & (y + dy >= 0) & (y + dy < n) &
+                      (z + dz >= 0) & (z + dz < n))
+                rr = r[ok]
+                if dyadic:
+                    v = (rng.integers(-64, 65, size=rr.size) / 64.0).astype(np.float32)
+                else:
+                    v = rng.standard_normal(rr.size).astype(np.float32)
+                absum[rr] += np.abs(v)
+                rows.append(rr.astype(np.int32))
+                cols.append((rr + dx + dy * n + dz * n * n).astype(np.int32))
+                vals.append(v)
+    centre = (absum + 1.0).astype(np.float32)
+    if dyadic:
+        centre = np.ceil(absum + 1.0).astype(np.float32)
+    rows.append(r.astype(np.int32))
+    cols.append(r.astype(np.int32))
+    vals.append(centre)
+    return _finish(np.concatenate(rows), np.concatenate(cols), np.concatenate(vals), m, m)
+
+
+def power_law(m: int = 1_000_000, seed: int = 2, s: float = 1.8, max_len: int = 5000,
+              band=(0, 1, -1, 1000, -1000)) -> Coo:
+    rng = np.random.default_rng(seed)
+    lengths = np.minimum(rng.zipf(s, size=m), max_len).astype(np.int64)
+    lengths = np.minimum(lengths, m)
+    rows = np.repeat(np.arange(m, dtype=np.int64), lengths)
+    cols = rng.integers(0, m, size=rows.size, dtype=np.int64)
+    vals = rng.standard_normal(rows.size).astype(np.float32)
+    brow, bcol = [], []
+    for off in band:
+        r = np.arange(max(0, -off), min(m, m - off), dtype=np.int64)
+        brow.append(r)
+        bcol.append(r + off)
+    brow = np.concatenate(brow)
+    bcol = np.concatenate(bcol)
+    bval = rng.standard_normal(brow.size).astype(np.float32)
+    # Band entries first so that they win over random duplicates.
+    return _finish(np.concatenate([brow, rows]).astype(np.int32),
+                   np.concatenate([bcol, cols]).astype(np.int32),
+                   np.concatenate([bval, vals]), m, m)
+
+
+BLOCK_OFFSETS = (0, 1, -1, 32, -32, 33, -33, 4096, -4096, 4097, -4097, 31)
+
+
+def block_stencil(nbr: int = 524_288, bh: int = 4, bw: int = 4, seed: int = 4,
+                  offsets=BLOCK_OFFSETS, dyadic: bool = False) -> Coo:
+    """Block-row pattern with len(offsets) dense bh x bw blocks per block row."""
+    rng = np.random.default_rng(seed)
+    nbc = nbr  # square in blocks
+    br = np.repeat(np.arange(nbr, dtype=np.int64), len(offsets))
+    bc = (br + np.tile(np.array(offsets, dtype=np.int64), nbr)) % nbc
+    hh, ww = np.meshgrid(np.arange(bh), np.arange(bw), indexing="ij")
+    rows = (br[:, None] * bh + hh.ravel()[None, :]).ravel()
+    cols = (bc[:, None] * bw + ww.ravel()[None, :]).ravel()
+    if dyadic:
+        vals = (rng.integers(-64, 65, size=rows.size) / 64.0).astype(np.float32)
+    else:
+        vals = rng.standard_normal(rows.size).astype(np.float32)
+    return _finish(rows.astype(np.int32), cols.astype(np.int32), vals, nbr * bh, nbc * bw)
+
+
+def random_sparse(height: int, width: int, density: float = 0.05, seed: int = 0,
+                  dyadic: bool = False, empty_rows: float = 0.0, long_rows: int = 0,
+                  long_len: int = 0) -> Coo:
+    rng = np.random.default_rng(seed)
+    nnz = int(height * width * density)
+    rows = rng.integers(0, height, size=nnz)
+    cols = rng.integers(0, width, size=nnz)
+    if empty_rows > 0:
+        dead = rng.random(height) < empty_rows
+        keep = ~dead[rows]
+        rows, cols = rows[keep], cols[keep]
+    if long_rows > 0 and long_len > 0:
+        lr = rng.choice(height, size=min(long_rows, height), replace=False)
+        extra_r = np.repeat(lr, long_len)
+        extra_c = rng.integers(0, width, size=extra_r.size)
+        rows = np.concatenate([rows, extra_r])
+        cols = np.concatenate([cols, extra_c])
+    if dyadic:
+        vals = (rng.integers(-64, 65, size=rows.size) / 64.0).astype(np.float32)
+    else:
+        vals = rng.standard_normal(rows.size).astype(np.float32)
+    return _finish(rows.astype(np.int32), cols.astype(np.int32), vals, height, width)
+
+
+def banded(n: int, offsets, seed: int = 0, dyadic: bool = False) -> Coo:
+    """Square n x n matrix holding full diagonals at the given offsets."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for off in offsets:
+        r = np.arange(max(0, -off), min(n, n - off), dtype=np.int64)
+        rows.append(r)
+        cols.append(r + off)
+    rows = np.concatenate(rows)
+    cols = np.concatenate(cols)
+    if dyadic:
+        vals = (rng.integers(-64, 65, size=rows.size) / 64.0).astype(np.float32)
+    else:
+        vals = rng.standard_normal(rows.size).astype(np.float32)
+    return _finish(rows.astype(np.int32), cols.astype(np.int32), vals, n, n)
+
+
+def dyadic_vector(n: int, seed: int = 1) -> np.ndarray:
+    """x with entries k/64, |k| <= 64: products with dyadic values k/64 and
+    their sums stay exact in FP32 while partial sums are below 2^24/4096."""
+    rng = np.random.default_rng(seed)
+    return (rng.integers(-64, 65, size=n) / 64.0).astype(np.float32)
+
+
+def uniform_vector(n: int, seed: int = 1) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1.0, 1.0, size=n).astype(np.float32)

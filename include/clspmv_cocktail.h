@@ -1,0 +1,164 @@
+/*
+ * clspmv_cocktail.h -- C-ABI of the Cocktail multi-format SpMV: the
+ * partitioner (analyze_matrix) and the device runtime that runs every part
+ * of a partitioned matrix into one y.
+ *
+ * Mirrors include/analyze.h (cocktail struct, analyze_matrix) and
+ * include/spmv_cocktail.h (init_mat_kernels, do_spmv, cpy_*, evaluate) of
+ * the reference.  OpenCL handles are replaced as follows:
+ *   cocktail_gpu&                      -> clspmv_cocktail_gpu*     (opaque)
+ *   cocktail_kernels& (+ its context
+ *   and command queue arguments)       -> clspmv_cocktail_kernels* (opaque;
+ *                                         owns one HIP stream on one GPU)
+ *   cl_mem                             -> float* device pointer
+ *   bool use_tex                       -> int, accepted and ignored (there
+ *                                         is no texture path on CDNA)
+ */
+#ifndef CLSPMV_COCKTAIL_H
+#define CLSPMV_COCKTAIL_H
+
+#include <stdbool.h>
+#include "clspmv_device.h"
+#include "clspmv_formats.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cocktail<int,int,float> (include/analyze.h:105-144), member for member.
+ * Part arrays are malloc()ed by clspmv_analyze_matrix and released by
+ * clspmv_free_cocktail. */
+typedef struct {
+    bool ifusebdia;
+    bool ifusedia;
+    bool ifusesbell;
+    bool ifusebell;
+    bool ifusebcsr;
+    bool ifusesell;
+    bool ifuseell;
+    bool ifusecsr;
+    bool ifusecoo;
+
+    int bdia_meth_num;
+    int dia_meth_num;
+    int sbell_meth_num;
+    int bell_meth_num;
+    int bcsr_meth_num;
+    int sell_meth_num;
+    int ell_meth_num;
+    int csr_meth_num;
+    int coo_meth_num;
+
+    int csr_group_num;
+    int coo_group_num;
+
+    int mat_width;
+    int mat_height;
+    int mat_nnz;
+
+    clspmv_bdia_matrix  bdia;
+    clspmv_dia_matrix   dia;
+    clspmv_sbell_matrix sbell;
+    clspmv_b4ell_matrix bell;
+    clspmv_b4csr_matrix bcsr;
+    clspmv_sell_matrix  sell;
+    clspmv_ell_matrix   ell;
+    clspmv_csr_matrix   csr;
+    clspmv_coo_matrix   coo;
+} clspmv_cocktail;
+
+/* init_cocktail / free_cocktail, include/analyze.h:146-206 */
+void clspmv_init_cocktail(clspmv_cocktail* mat);
+void clspmv_free_cocktail(clspmv_cocktail* mat);
+
+/* analyze_matrix, include/analyze.h:257 / src/analyze.cpp:3714.
+ * Splits coomat into at most nine parts (BDIA, DIA, SBELL, BELL, BCSR,
+ * SELL, ELL, CSR, COO) by the reference's cost model over per-format
+ * GFLOP/s tables.  The tables are read from <ben_dir>/benchmark/<fmt>.ben
+ * (the reference reads $CLSPMVPATH/benchmark, src/analyze.cpp:179); a NULL
+ * ben_dir falls back to $CLSPMVPATH.  coomat must be sorted by (row, col)
+ * without duplicates.  Returns CLSPMV_OK or an error (missing or empty
+ * tables are reported instead of read out of bounds). */
+int clspmv_analyze_matrix(clspmv_coo_matrix* coomat, clspmv_cocktail* mat,
+                          float dia_priority_multiplier,
+                          int estimate_flat_partial, int estimate_flat_full,
+                          const char* ben_dir);
+
+/* Optional log of the partitioner's decisions (one line per decision);
+ * NULL disables.  Not thread safe. */
+void clspmv_set_analyze_log(void (*log)(const char* line));
+
+/* ---- device runtime, include/spmv_cocktail.h:166-175 ------------------- */
+typedef struct clspmv_cocktail_gpu clspmv_cocktail_gpu;
+typedef struct clspmv_cocktail_kernels clspmv_cocktail_kernels;
+
+/* init_mat_kernels (src/spmv_cocktail.cpp:1590): uploads every part of
+ * cpumat to device `device_ordinal`, x from vec (mat_width floats) and y
+ * from res (mat_height floats), and prepares the launch sequence, captured
+ * once into a HIP graph.  The cocktail is not retained. */
+int clspmv_init_mat_kernels(const clspmv_cocktail* cpumat, const float* vec,
+                            const float* res, clspmv_cocktail_gpu** gpumat,
+                            clspmv_cocktail_kernels** kernels, int use_tex);
+/* Same with an explicit device. */
+int clspmv_init_mat_kernels_on(int device_ordinal, const clspmv_cocktail* cpumat,
+                               const float* vec, const float* res,
+                               clspmv_cocktail_gpu** gpumat,
+                               clspmv_cocktail_kernels** kernels);
+
+/* do_spmv (src/spmv_cocktail.cpp:1088): ntimes passes of y += A*x over all
+ * parts in the reference's order BDIA, DIA, SBELL, BELL, BCSR, SELL, ELL,
+ * CSR, COO; blocks until done. */
+int clspmv_do_spmv(clspmv_cocktail_kernels* kernels, int ntimes);
+/* Same without the final wait (work stays queued on the kernels' stream). */
+int clspmv_do_spmv_async(clspmv_cocktail_kernels* kernels, int ntimes);
+
+/* cpy_* (src/spmv_cocktail.cpp:1459-1587) */
+int clspmv_cpy_vector_from_cpu(clspmv_cocktail_gpu* gpumat, clspmv_cocktail_kernels* kernels, const float* vec, int vec_size);
+int clspmv_cpy_vector_from_gpu(clspmv_cocktail_gpu* gpumat, clspmv_cocktail_kernels* kernels, const float* d_vec, int vec_size);
+int clspmv_cpy_result_from_cpu(clspmv_cocktail_gpu* gpumat, clspmv_cocktail_kernels* kernels, const float* res, int res_size);
+int clspmv_cpy_result_from_gpu(clspmv_cocktail_gpu* gpumat, clspmv_cocktail_kernels* kernels, const float* d_res, int res_size);
+int clspmv_cpy_result_to_cpu(clspmv_cocktail_gpu* gpumat, clspmv_cocktail_kernels* kernels, float* res, int res_size);
+int clspmv_cpy_result_to_gpu(clspmv_cocktail_gpu* gpumat, clspmv_cocktail_kernels* kernels, float* d_res, int res_size);
+
+/* free_mat_kernels (src/spmv_cocktail.cpp:1613) */
+int clspmv_free_mat_kernels(clspmv_cocktail_gpu* gpumat, clspmv_cocktail_kernels* kernels);
+
+/* gpumat.vec / gpumat.res of the reference: device x and y. */
+float* clspmv_cocktail_gpu_vec(clspmv_cocktail_gpu* gpumat);
+float* clspmv_cocktail_gpu_res(clspmv_cocktail_gpu* gpumat);
+/* The context (device + stream) the kernels run on. */
+clspmv_ctx* clspmv_cocktail_kernels_ctx(clspmv_cocktail_kernels* kernels);
+
+/* Per-part timing: each part's kernels timed alone with HIP events over
+ * ntimes launches, then the whole captured pass.  Parts are reported in
+ * launch order. */
+typedef struct {
+    int    format;     /* enum clspmv_format                   */
+    int    method;
+    int    nnz;        /* true nonzeros of the part            */
+    long long stored;  /* stored entries of the part           */
+    double ms;         /* average per launch of this part      */
+} clspmv_part_timing;
+
+int clspmv_cocktail_profile(clspmv_cocktail_kernels* kernels, int ntimes,
+                            clspmv_part_timing* parts, int max_parts,
+                            int* nparts, double* graph_ms);
+
+/* evaluate (src/spmv_cocktail.cpp:1431): runs one SpMV from res, compares
+ * with coores (the caller's expected y, e.g. from a serial COO SpMV) using
+ * the reference's report (src/util.cpp:93), then times 1000 passes and
+ * prints "Cocktail ... GFLOPS" with GFLOP/s = 2*nnz/t. */
+typedef struct {
+    double ms_per_spmv;
+    double gflops;
+    double l2_distance;
+    double max_diff;
+    double max_ratio;
+} clspmv_eval_report;
+int clspmv_evaluate(const clspmv_cocktail* cpumat, const float* vec, float* res,
+                    const float* coores, clspmv_eval_report* report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLSPMV_COCKTAIL_H */

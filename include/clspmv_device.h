@@ -1,0 +1,151 @@
+/*
+ * clspmv_device.h -- C-ABI of the MI355X device runtime: contexts (one HIP
+ * stream on one GPU), device-resident matrices of every storage format, the
+ * per-format SpMV entry points and the per-format benchmark drivers.
+ *
+ * Every SpMV computes y += A*x (the reference's contract: each kernel
+ * accumulates into result, kernels/spmv_*.cl; the caller preloads y,
+ * src/eval.cpp:40-42).  Device pointers are plain float* obtained from
+ * clspmv_device_malloc, hipMalloc or any other allocator of the same device.
+ * Launch entry points are asynchronous on the context's stream; the driver
+ * entry points block, like the reference's clFinish-terminated calls.
+ */
+#ifndef CLSPMV_DEVICE_H
+#define CLSPMV_DEVICE_H
+
+#include <stddef.h>
+#include "clspmv_formats.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- contexts: replace cl_context + cl_command_queue -------------------
+ * (include/oclcommon.h, src/oclcommon.cpp:259 initialization()).  One
+ * context = one device + one in-order stream at default priority. */
+typedef struct clspmv_ctx clspmv_ctx;
+int   clspmv_ctx_create(int device_ordinal, clspmv_ctx** out);
+int   clspmv_ctx_destroy(clspmv_ctx* ctx);
+int   clspmv_ctx_synchronize(clspmv_ctx* ctx);
+void* clspmv_ctx_stream(clspmv_ctx* ctx);       /* the hipStream_t */
+int   clspmv_ctx_device(clspmv_ctx* ctx);
+/* Name and compute-unit count of the context's device. */
+int   clspmv_device_info(clspmv_ctx* ctx, char* name, int name_len, int* cu_count);
+
+int clspmv_device_malloc(clspmv_ctx* ctx, size_t bytes, void** out);
+int clspmv_device_free(clspmv_ctx* ctx, void* ptr);
+int clspmv_memcpy_h2d(clspmv_ctx* ctx, void* dst, const void* src, size_t bytes);
+int clspmv_memcpy_d2h(clspmv_ctx* ctx, void* dst, const void* src, size_t bytes);
+int clspmv_memcpy_d2d(clspmv_ctx* ctx, void* dst, const void* src, size_t bytes);
+int clspmv_memset_zero(clspmv_ctx* ctx, void* dst, size_t bytes);
+
+/* ---- device-resident matrices ------------------------------------------ */
+enum clspmv_format {
+    CLSPMV_FMT_CSR   = 0,   /* csr_matrix   -> kernels/spmv_csr_*.cl   */
+    CLSPMV_FMT_COO   = 1,   /* coo_matrix   -> kernels/spmv_coo.cl     */
+    CLSPMV_FMT_ELL   = 2,   /* ell_matrix   -> kernels/spmv_ell.cl     */
+    CLSPMV_FMT_SELL  = 3,   /* sell_matrix  -> kernels/spmv_sell.cl    */
+    CLSPMV_FMT_DIA   = 4,   /* dia_matrix   -> kernels/spmv_dia.cl     */
+    CLSPMV_FMT_BDIA  = 5,   /* bdia_matrix  -> kernels/spmv_bdia.cl    */
+    CLSPMV_FMT_BCSR  = 6,   /* b4csr_matrix -> kernels/spmv_bcsr.cl    */
+    CLSPMV_FMT_BELL  = 7,   /* b4ell_matrix -> kernels/spmv_bell.cl    */
+    CLSPMV_FMT_SBELL = 8    /* sbell_matrix -> kernels/spmv_sbell.cl   */
+};
+
+/* Kernel variants ("methods") per format.  Method numbers index the
+ * per-format GFLOP/s columns of the .ben tables read by the Cocktail
+ * partitioner, so the counts equal the reference's *_IMP_NUM.
+ *   CSR  0: vector kernel, lanes per row = power of two near nnz/row
+ *        1: adaptive row blocks (short rows share an LDS-staged block,
+ *           long rows take a whole work-group)
+ *   COO  0: one-pass segmented reduction + carry fix-up
+ *   ELL  0: one row per lane          1: four rows per lane (128-bit loads)
+ *   SELL 0/1: one row per lane (slice height is a property of the data)
+ *   DIA  0: row per lane              1: four rows per lane
+ *        2: row per lane, x window in LDS   3: four rows, x window in LDS
+ *   BDIA 0: row per lane              1: row per lane, LDS x window per band
+ *        2: two rows per lane, LDS    3: four rows per lane
+ *        4: four rows per lane, LDS x window per band
+ *   BCSR 0: lane group per block row (group size from blocks per row)
+ *   BELL 0: lane per block row, scalar FMA   1: float4 accumulators
+ *   SBELL 0: lane per block row within a slice                       */
+#define CLSPMV_METHOD_AUTO (-1)
+int clspmv_method_count(int format);
+
+typedef struct clspmv_dmat clspmv_dmat;
+
+typedef struct {
+    int    format;          /* enum clspmv_format                       */
+    int    rows, cols;      /* matinfo.height, matinfo.width            */
+    int    nnz;             /* true nonzeros (matinfo.nnz)              */
+    long long stored;       /* stored entries incl. padding             */
+    size_t device_bytes;    /* device memory held by the matrix         */
+    int    bheight, bwidth; /* block shape (block formats), else 1,1    */
+    int    slice_height;    /* SELL / SBELL slice height, else 0        */
+} clspmv_dmat_info;
+
+/* Upload a host matrix (the reference's cpy_<fmt>_mat,
+ * src/spmv_cocktail.cpp:304-962).  The host struct is not retained. */
+int clspmv_upload_csr(clspmv_ctx* ctx, const clspmv_csr_matrix* m, clspmv_dmat** out);
+int clspmv_upload_coo(clspmv_ctx* ctx, const clspmv_coo_matrix* m, clspmv_dmat** out);
+int clspmv_upload_ell(clspmv_ctx* ctx, const clspmv_ell_matrix* m, clspmv_dmat** out);
+int clspmv_upload_sell(clspmv_ctx* ctx, const clspmv_sell_matrix* m, clspmv_dmat** out);
+int clspmv_upload_dia(clspmv_ctx* ctx, const clspmv_dia_matrix* m, clspmv_dmat** out);
+int clspmv_upload_bdia(clspmv_ctx* ctx, const clspmv_bdia_matrix* m, clspmv_dmat** out);
+int clspmv_upload_b4csr(clspmv_ctx* ctx, const clspmv_b4csr_matrix* m, clspmv_dmat** out);
+int clspmv_upload_b4ell(clspmv_ctx* ctx, const clspmv_b4ell_matrix* m, clspmv_dmat** out);
+int clspmv_upload_sbell(clspmv_ctx* ctx, const clspmv_sbell_matrix* m, clspmv_dmat** out);
+int clspmv_dmat_free(clspmv_dmat* m);
+int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out);
+
+/* y += A*x on the context's stream (asynchronous).  x holds cols floats,
+ * y holds rows floats.  method = CLSPMV_METHOD_AUTO picks by shape. */
+int clspmv_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
+                const float* d_x, float* d_y);
+
+/* Times ntimes back-to-back launches of clspmv_spmv after warmup
+ * untimed ones, with HIP events on the context's stream; y keeps
+ * accumulating.  Blocks.  *ms_per_call = average kernel-to-kernel time. */
+int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
+                     const float* d_x, float* d_y, int warmup, int ntimes,
+                     double* ms_per_call);
+
+/* ---- per-format drivers (include/spmv_<fmt>.h:7) ------------------------
+ * spmv_<fmt>(char* oclfilename, coo_matrix<int,float>* mat, int dim2Size,
+ *            int ntimes, cl_device_type) -- the reference converts the COO
+ * matrix, runs every kernel variant, times ntimes launches of each and
+ * prints the best (e.g. src/spmv_bdia.cpp:413-440).  Here kernel_file is
+ * accepted and ignored (kernels are compiled in), device_ordinal replaces
+ * the OpenCL device type, and the outcome is also returned in report.
+ * y_out (optional, rows floats) receives A*x of the best variant for
+ * x[i] = 1 (the all-ones vector of src/eval.cpp:119), y starting at zero.
+ * Block formats sweep the eight reference block shapes. */
+typedef struct {
+    int    format;
+    int    variants;              /* variants run                        */
+    int    best_method;
+    int    best_bheight, best_bwidth;
+    double best_ms;               /* per launch                          */
+    double best_gflops;           /* 2*nnz / t                           */
+    double ms[64];                /* per variant, in run order           */
+    int    method[64];
+    int    bheight[64], bwidth[64];
+    double max_rel_diff;          /* largest relative difference between
+                                     any variant's y and the first's    */
+} clspmv_driver_report;
+
+int clspmv_spmv_csr_scalar(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+int clspmv_spmv_csr_vector(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+int clspmv_spmv_coo(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+int clspmv_spmv_ell(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+int clspmv_spmv_sell(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+int clspmv_spmv_dia(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+int clspmv_spmv_bdia(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+int clspmv_spmv_bell(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+int clspmv_spmv_sbell(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+int clspmv_spmv_bcsr(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLSPMV_DEVICE_H */

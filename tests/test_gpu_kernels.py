@@ -1,0 +1,201 @@
+"""GPU parity of every storage format and kernel variant against the oracle.
+
+Bar: |y_i - y64_i| <= 1e-5 * (|y0_i| + sum_j |a_ij x_j|) against the FP64
+oracle, and bit-exact equality on dyadic inputs,
+where every FP32 partial sum is exact so the summation order cannot matter.
+All calls go through the C-ABI (libclspmv.so).
+"""
+
+import numpy as np
+import pytest
+
+import clspmv_mic_amd as cs
+from clspmv_mic_amd import matrices
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _cases():
+    return {
+        "rand_small": matrices.random_sparse(300, 257, 0.03, seed=1),
+        "rand_dyadic": matrices.random_sparse(513, 700, 0.02, seed=2, dyadic=True),
+        "rand_empty_rows": matrices.random_sparse(1000, 1000, 0.01, seed=3, empty_rows=0.3),
+        "rand_long_rows": matrices.random_sparse(2000, 3000, 0.002, seed=4, long_rows=5,
+                                                 long_len=2500),
+        "rand_long_dyadic": matrices.random_sparse(1500, 1500, 0.003, seed=5, dyadic=True,
+                                                   long_rows=3, long_len=1400),
+        "lap2d_256": matrices.laplacian_2d(256),
+        "stencil27_16": matrices.stencil_27pt(16, seed=3),
+        "stencil27_dyadic": matrices.stencil_27pt(12, seed=5, dyadic=True),
+        "block_small": matrices.block_stencil(700, 4, 4, seed=6, offsets=(0, 1, -1, 32, -33, 5)),
+        "tall": matrices.random_sparse(3001, 97, 0.05, seed=7),
+        "wide": matrices.random_sparse(97, 3001, 0.05, seed=8),
+        "banded": matrices.banded(5000, (0, 1, -1, 2, 3, -7, 300, -300, 301, 302), seed=9),
+        "banded_dyadic": matrices.banded(4099, (-2, -1, 0, 1, 5, 6, 7, 8, 1000), seed=10,
+                                         dyadic=True),
+    }
+
+
+CASES = _cases()
+
+
+def _is_dyadic(a: np.ndarray) -> bool:
+    return bool(np.all(a * 64 == np.round(a * 64)))
+
+
+def _run(ctx, oracle, A: cs.Coo, owned, method):
+    dyadic = _is_dyadic(A.val)
+    if dyadic:
+        x = matrices.dyadic_vector(A.width, seed=11)
+        y0 = matrices.dyadic_vector(A.height, seed=12)
+    else:
+        x = matrices.uniform_vector(A.width, seed=11)
+        y0 = matrices.uniform_vector(A.height, seed=12)
+    dA = cs.DeviceMatrix(ctx, owned)
+    dx = cs.DeviceVector(ctx, A.width, x)
+    dy = cs.DeviceVector(ctx, A.height, y0)
+    dA.spmv(dx, dy, method)
+    ctx.sync()
+    y = dy.download()
+    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+    err, worst = oracle.scaled_error(y, y64, bound)
+    assert err <= TOL, f"scaled error {err:.3e} at row {worst}: got {y[worst]} want {y64[worst]}"
+    if dyadic:
+        np.testing.assert_array_equal(y, y64.astype(np.float32))
+    return y
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("method", [0, 1])
+def test_csr(ctx, oracle, name, method):
+    A = CASES[name]
+    _run(ctx, oracle, A, cs.coo2csr(A.copy()), method)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_coo(ctx, oracle, name):
+    A = CASES[name]
+    _run(ctx, oracle, A, A.copy(), 0)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("method", [0, 1])
+def test_ell(ctx, oracle, name, method):
+    A = CASES[name]
+    if name.startswith("rand_long"):
+        pytest.skip("ELL of a matrix with 2500-entry rows is all padding")
+    _run(ctx, oracle, A, cs.coo2ell(A.copy()), method)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("slice_height", [32, 128])
+def test_sell(ctx, oracle, name, slice_height):
+    A = CASES[name]
+    for method in (0, 1):
+        _run(ctx, oracle, A, cs.coo2sell(A, slice_height), method)
+
+
+DIA_CASES = ["lap2d_256", "stencil27_16", "stencil27_dyadic", "block_small", "banded",
+             "banded_dyadic"]
+
+
+@pytest.mark.parametrize("name", DIA_CASES)
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+def test_dia(ctx, oracle, name, method):
+    A = CASES[name]
+    d = cs.coo2dia(A)
+    if d is None:
+        pytest.skip("DIA refused: padded size >= 2*nnz (include/matrix_storage.h:972)")
+    offs = d.arrays()["off"]
+    span = int(offs.max() - offs.min())
+    if method >= 2 and (1024 if method == 3 else 256) + span > 8192:
+        with pytest.raises(cs.ClspmvError):
+            _run(ctx, oracle, A, d, method)
+        return
+    _run(ctx, oracle, A, d, method)
+
+
+@pytest.mark.parametrize("name", DIA_CASES)
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
+def test_bdia(ctx, oracle, name, method):
+    A = CASES[name]
+    b = cs.coo2bdia(A)
+    if b is None:
+        pytest.skip("BDIA refused: padded size >= 2*nnz (include/matrix_storage.h:972)")
+    _run(ctx, oracle, A, b, method)
+
+
+BLOCK_CASES = ["block_small", "rand_small", "rand_dyadic", "lap2d_256", "tall", "wide"]
+
+
+@pytest.mark.parametrize("name", BLOCK_CASES)
+@pytest.mark.parametrize("shape", cs.BLOCK_SHAPES)
+def test_bcsr(ctx, oracle, name, shape):
+    A = CASES[name]
+    bh, bw = shape
+    _run(ctx, oracle, A, cs.coo2b4csr(A.copy(), bw, bh), 0)
+
+
+@pytest.mark.parametrize("name", BLOCK_CASES)
+@pytest.mark.parametrize("shape", cs.BLOCK_SHAPES)
+def test_bell(ctx, oracle, name, shape):
+    A = CASES[name]
+    bh, bw = shape
+    owned = cs.coo2b4ell(A.copy(), bw, bh)
+    for method in (0, 1):
+        _run(ctx, oracle, A, owned, method)
+
+
+@pytest.mark.parametrize("name", BLOCK_CASES)
+@pytest.mark.parametrize("shape", cs.BLOCK_SHAPES)
+def test_sbell(ctx, oracle, name, shape):
+    A = CASES[name]
+    bh, bw = shape
+    _run(ctx, oracle, A, cs.coo2sbell(A.copy(), bw, bh), 0)
+
+
+def test_auto_method_every_format(ctx, oracle):
+    A = CASES["stencil27_16"]
+    for owned in [cs.coo2csr(A.copy()), A.copy(), cs.coo2ell(A.copy()), cs.coo2sell(A),
+                  cs.coo2dia(A), cs.coo2bdia(A), cs.coo2b4csr(A.copy(), 4, 4),
+                  cs.coo2b4ell(A.copy(), 4, 4), cs.coo2sbell(A.copy(), 4, 4)]:
+        _run(ctx, oracle, A, owned, cs.METHOD_AUTO)
+
+
+def test_empty_matrix(ctx):
+    A = cs.Coo(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32), 10, 10)
+    dA = cs.DeviceMatrix(ctx, cs.coo2csr(A))
+    y0 = np.arange(10, dtype=np.float32)
+    dx = cs.DeviceVector(ctx, 10, np.ones(10, np.float32))
+    dy = cs.DeviceVector(ctx, 10, y0)
+    dA.spmv(dx, dy, 0)
+    ctx.sync()
+    np.testing.assert_array_equal(dy.download(), y0)
+
+
+def test_bad_method_rejected(ctx):
+    A = CASES["rand_small"]
+    dA = cs.DeviceMatrix(ctx, cs.coo2csr(A.copy()))
+    dx = cs.DeviceVector(ctx, A.width, np.ones(A.width, np.float32))
+    dy = cs.DeviceVector(ctx, A.height, np.zeros(A.height, np.float32))
+    with pytest.raises(cs.ClspmvError):
+        dA.spmv(dx, dy, 7)
+
+
+def test_repeat_is_deterministic(ctx):
+    """Same inputs, same kernel: bitwise identical results (no atomics)."""
+    A = CASES["rand_long_rows"]
+    x = matrices.uniform_vector(A.width, seed=3)
+    for owned, m in [(cs.coo2csr(A.copy()), 1), (A.copy(), 0)]:
+        dA = cs.DeviceMatrix(ctx, owned)
+        dx = cs.DeviceVector(ctx, A.width, x)
+        outs = []
+        for _ in range(3):
+            dy = cs.DeviceVector(ctx, A.height, np.zeros(A.height, np.float32))
+            dA.spmv(dx, dy, m)
+            ctx.sync()
+            outs.append(dy.download())
+        np.testing.assert_array_equal(outs[0], outs[1])
+        np.testing.assert_array_equal(outs[0], outs[2])
