@@ -234,6 +234,7 @@ _sig("clspmv_dmat_free", C.c_int, VP)
 _sig("clspmv_dmat_get_info", C.c_int, VP, P(DmatInfo))
 _sig("clspmv_spmv", C.c_int, VP, VP, C.c_int, VP, VP)
 _sig("clspmv_time_spmv", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int, C.c_int, P(C.c_double))
+_sig("clspmv_bandwidth_probe", C.c_int, VP, C.c_size_t, C.c_int, P(C.c_double), P(C.c_double))
 DRIVERS = ["csr_scalar", "csr_vector", "coo", "ell", "sell", "dia", "bdia", "bell", "sbell", "bcsr"]
 for _n in DRIVERS:
     _sig(f"clspmv_spmv_{_n}", C.c_int, C.c_char_p, P(CooMatrix), C.c_int, C.c_int, C.c_int,
@@ -499,6 +500,12 @@ class Context:
     def sync(self):
         check(lib.clspmv_ctx_synchronize(self.h), "ctx_synchronize")
 
+    def bandwidth_probe(self, nbytes: int = 1 << 30, ntimes: int = 20):
+        """(ms per copy, GB/s counting read + write) of a float4 copy."""
+        ms, gbps = C.c_double(), C.c_double()
+        check(lib.clspmv_bandwidth_probe(self.h, nbytes, ntimes, C.byref(ms), C.byref(gbps)), "probe")
+        return ms.value, gbps.value
+
     def info(self):
         buf = C.create_string_buffer(256)
         cu = C.c_int()
@@ -587,3 +594,41 @@ def run_driver(name: str, coo: Coo, ntimes: int = 10, device: int = 0, want_y: b
     check(getattr(lib, f"clspmv_spmv_{name}")(None, coo.ref(), 1, ntimes, device, C.byref(rep),
                                                _fp(y) if want_y else None), f"spmv_{name}")
     return rep, y
+
+
+# ---------------------------------------------------------------------------
+# Cocktail (include/clspmv_cocktail.h)
+
+class Cocktail:
+    """A partitioned matrix (cocktail<int,int,float>) owned by the library."""
+
+    def __init__(self):
+        self.c = CocktailMatrix()
+        lib.clspmv_init_cocktail(C.byref(self.c))
+
+    def __del__(self):
+        try:
+            lib.clspmv_free_cocktail(C.byref(self.c))
+        except Exception:
+            pass
+
+    def parts(self) -> list:
+        """[(format name, part struct)] of the parts in use, in launch order."""
+        out = []
+        for n in ["bdia", "dia", "sbell", "bell", "bcsr", "sell", "ell", "csr", "coo"]:
+            if getattr(self.c, "ifuse" + n):
+                out.append((n, getattr(self.c, n)))
+        return out
+
+    def summary(self) -> dict:
+        return {n: int(getattr(p, "matinfo").nnz) for n, p in self.parts()}
+
+
+def analyze_matrix(coo: Coo, ben_dir: Optional[str] = None, dia_priority_multiplier: float = 1.0,
+                   estimate_flat_partial: bool = True, estimate_flat_full: bool = False) -> Cocktail:
+    """analyze_matrix (include/analyze.h:257); tables from <ben_dir>/benchmark."""
+    ck = Cocktail()
+    check(lib.clspmv_analyze_matrix(coo.ref(), C.byref(ck.c), C.c_float(dia_priority_multiplier),
+                                    int(estimate_flat_partial), int(estimate_flat_full),
+                                    ben_dir.encode() if ben_dir else None), "analyze_matrix")
+    return ck

@@ -56,5 +56,9 @@ hipError_t sbell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
                  int slice_height, const int* sptr, const int* col4,
                  const float* data, const float* x, bool x_aligned, float* y);
 
+// ---- bandwidth probe ---------------------------------------------------------
+// dst[0..4*n4) = src[0..4*n4) with 128-bit loads/stores (16-byte aligned).
+hipError_t copy4(hipStream_t s, long long n4, const float* src, float* dst);
+
 }  // namespace launch
 }  // namespace clspmv

@@ -572,4 +572,34 @@ int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
     return CLSPMV_OK;
 }
 
+int clspmv_bandwidth_probe(clspmv_ctx* ctx, size_t bytes, int ntimes,
+                           double* ms_per_copy, double* gbps) {
+    if (!ctx || ntimes <= 0 || bytes < 16) return fail(CLSPMV_ERR_INVALID, "bad argument");
+    int st = use_ctx_device(ctx);
+    if (st) return st;
+    const long long n4 = (long long)(bytes / 16);
+    float *a = nullptr, *b = nullptr;
+    CLSPMV_HIP(hipMalloc(&a, n4 * 16));
+    hipError_t e = hipMalloc(&b, n4 * 16);
+    if (e != hipSuccess) {
+        hipFree(a);
+        return fail(CLSPMV_ERR_ALLOC, "probe buffer: %s", hipGetErrorString(e));
+    }
+    float ms = 0.0f;
+    e = hipMemsetAsync(a, 0, n4 * 16, ctx->stream);
+    if (e == hipSuccess) e = launch::copy4(ctx->stream, n4, a, b);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_start, ctx->stream);
+    for (int i = 0; i < ntimes && e == hipSuccess; ++i) e = launch::copy4(ctx->stream, n4, a, b);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "probe failed: %s", hipGetErrorString(e));
+    const double per = (double)ms / ntimes;
+    if (ms_per_copy) *ms_per_copy = per;
+    if (gbps) *gbps = per > 0 ? 2.0 * n4 * 16 / (per * 1e-3) / 1e9 : 0.0;
+    return CLSPMV_OK;
+}
+
 }  // extern "C"

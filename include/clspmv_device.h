@@ -110,6 +110,13 @@ int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
                      const float* d_x, float* d_y, int warmup, int ntimes,
                      double* ms_per_call);
 
+/* ---- bandwidth probe (src/mem_bandwidth.cpp bandwidth_test :14) ---------
+ * Copies `bytes` (rounded down to 16) between two fresh device buffers
+ * `ntimes` times after one warm-up copy and returns the average copy time
+ * and GB/s counted as bytes read + bytes written. */
+int clspmv_bandwidth_probe(clspmv_ctx* ctx, size_t bytes, int ntimes,
+                           double* ms_per_copy, double* gbps);
+
 /* ---- per-format drivers (include/spmv_<fmt>.h:7) ------------------------
  * spmv_<fmt>(char* oclfilename, coo_matrix<int,float>* mat, int dim2Size,
  *            int ntimes, cl_device_type) -- the reference converts the COO

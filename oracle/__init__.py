@@ -125,3 +125,42 @@ class Reference:
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing (needs the reference tree): make -C oracle ref")
         self.lib = C.CDLL(path)
+        import clspmv_mic_amd as cs
+        self.cs = cs
+        P = C.POINTER
+        L = self.lib
+        L.ref_analyze_matrix.argtypes = [P(cs.CooMatrix), P(cs.CocktailMatrix), C.c_float, C.c_int,
+                                         C.c_int, C.c_char_p]
+        L.ref_free_cocktail.argtypes = [P(cs.CocktailMatrix)]
+
+    def analyze(self, coo, ben_dir: str, mult=1.0, partial=True, full=False):
+        """The reference's analyze_matrix; returns a CocktailMatrix whose arrays
+        belong to the reference (release with free_cocktail)."""
+        c = self.cs.CocktailMatrix()
+        self.lib.ref_analyze_matrix(C.byref(coo.c), C.byref(c), C.c_float(mult), int(partial),
+                                    int(full), ben_dir.encode())
+        return c
+
+    def free_cocktail(self, c):
+        self.lib.ref_free_cocktail(C.byref(c))
+
+    def convert(self, name: str, coo, *args):
+        """ref_coo2<name>(coo, dst, *args); returns (status, struct)."""
+        cs = self.cs
+        T = {"csr": cs.CsrMatrix, "ell": cs.EllMatrix, "sell": cs.SellMatrix, "dia": cs.DiaMatrix,
+             "bdia": cs.BdiaMatrix, "bcsr": cs.BcsrMatrix, "b4csr": cs.B4csrMatrix,
+             "bell": cs.BellMatrix, "b4ell": cs.B4ellMatrix, "sbell": cs.SbellMatrix}[name]
+        dst = T()
+        f = getattr(self.lib, f"ref_coo2{name}")
+        f.restype = C.c_int
+        # Block shapes and slice heights are unsigned, alignments and widths int.
+        unsigned = {"sell": 1, "bcsr": 2, "b4csr": 2, "bell": 2, "b4ell": 2, "sbell": 3}.get(name, 0)
+        conv = [C.c_uint(a) if i < unsigned else C.c_int(a) for i, a in enumerate(args)]
+        rc = f(C.byref(coo.c), C.byref(dst), *conv)
+        return rc, dst
+
+    def spmv(self, name: str, struct, x, y):
+        """The reference's serial <name>_spmv (include/spmv_serial.h) on a struct."""
+        f = getattr(self.lib, f"ref_{name}_spmv")
+        f.restype = None
+        f(C.byref(struct), x.ctypes.data_as(PF), y.ctypes.data_as(PF))
