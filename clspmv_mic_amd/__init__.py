@@ -245,6 +245,7 @@ if hasattr(lib, "clspmv_analyze_matrix"):
     _sig("clspmv_free_cocktail", None, P(CocktailMatrix))
     _sig("clspmv_analyze_matrix", C.c_int, P(CooMatrix), P(CocktailMatrix), C.c_float, C.c_int,
          C.c_int, C.c_char_p)
+_sig("clspmv_autotune", C.c_int, C.c_int, C.c_char_p, C.c_int, C.c_int)
 if hasattr(lib, "clspmv_init_mat_kernels"):
     _sig("clspmv_init_mat_kernels", C.c_int, P(CocktailMatrix), PF, PF, P(VP), P(VP), C.c_int)
     _sig("clspmv_init_mat_kernels_on", C.c_int, C.c_int, P(CocktailMatrix), PF, PF, P(VP), P(VP))
@@ -624,6 +625,16 @@ class Cocktail:
         return {n: int(getattr(p, "matinfo").nnz) for n, p in self.parts()}
 
 
+def autotune(out_dir: str, device: int = 0, level: int = 0, verbose: bool = False) -> str:
+    """Measure all kernel variants on this GPU and write <out_dir>/benchmark/*.ben."""
+    check(lib.clspmv_autotune(device, out_dir.encode(), level, int(verbose)), "autotune")
+    return out_dir
+
+
+# Tables measured on an MI355X by autotune(), shipped with the package.
+TABLES_DIR = os.path.join(_HERE, "tables_mi355x")
+
+
 def analyze_matrix(coo: Coo, ben_dir: Optional[str] = None, dia_priority_multiplier: float = 1.0,
                    estimate_flat_partial: bool = True, estimate_flat_full: bool = False) -> Cocktail:
     """analyze_matrix (include/analyze.h:257); tables from <ben_dir>/benchmark."""
@@ -632,3 +643,63 @@ def analyze_matrix(coo: Coo, ben_dir: Optional[str] = None, dia_priority_multipl
                                     int(estimate_flat_partial), int(estimate_flat_full),
                                     ben_dir.encode() if ben_dir else None), "analyze_matrix")
     return ck
+
+
+class CocktailRuntime:
+    """init_mat_kernels ... free_mat_kernels (include/spmv_cocktail.h:166-175)
+    over a partitioned matrix; one HIP graph per pass over the parts."""
+
+    def __init__(self, ck: Cocktail, x: np.ndarray, y: Optional[np.ndarray] = None, device: int = 0):
+        x = np.ascontiguousarray(x, np.float32)
+        self.width, self.height, self.nnz = ck.c.mat_width, ck.c.mat_height, ck.c.mat_nnz
+        y = np.zeros(self.height, np.float32) if y is None else np.ascontiguousarray(y, np.float32)
+        g, k = VP(), VP()
+        check(lib.clspmv_init_mat_kernels_on(device, C.byref(ck.c), _fp(x), _fp(y), C.byref(g), C.byref(k)),
+              "init_mat_kernels")
+        self.g, self.k = g, k
+
+    def do_spmv(self, ntimes: int = 1):
+        check(lib.clspmv_do_spmv(self.k, ntimes), "do_spmv")
+
+    def do_spmv_async(self, ntimes: int = 1):
+        check(lib.clspmv_do_spmv_async(self.k, ntimes), "do_spmv_async")
+
+    def sync(self):
+        check(lib.clspmv_ctx_synchronize(lib.clspmv_cocktail_kernels_ctx(self.k)), "sync")
+
+    def set_x(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        check(lib.clspmv_cpy_vector_from_cpu(self.g, self.k, _fp(x), x.size), "cpy_vector_from_cpu")
+
+    def set_y(self, y: np.ndarray):
+        y = np.ascontiguousarray(y, np.float32)
+        check(lib.clspmv_cpy_result_from_cpu(self.g, self.k, _fp(y), y.size), "cpy_result_from_cpu")
+
+    def get_y(self) -> np.ndarray:
+        y = np.empty(self.height, np.float32)
+        check(lib.clspmv_cpy_result_to_cpu(self.g, self.k, _fp(y), self.height), "cpy_result_to_cpu")
+        return y
+
+    def y_to_x(self):
+        """cpy_vector_from_gpu(gpumat, ..., gpumat.res): next x = current y."""
+        check(lib.clspmv_cpy_vector_from_gpu(self.g, self.k, lib.clspmv_cocktail_gpu_res(self.g),
+                                             self.height), "cpy_vector_from_gpu")
+
+    def profile(self, ntimes: int = 20):
+        parts = (PartTiming * 16)()
+        n = C.c_int()
+        gms = C.c_double()
+        check(lib.clspmv_cocktail_profile(self.k, ntimes, parts, 16, C.byref(n), C.byref(gms)), "profile")
+        return [dict(format=FORMAT_NAMES[parts[i].format], method=parts[i].method, nnz=parts[i].nnz,
+                     stored=parts[i].stored, ms=parts[i].ms) for i in range(n.value)], gms.value
+
+    def close(self):
+        if self.k:
+            lib.clspmv_free_mat_kernels(self.g, self.k)
+            self.g, self.k = VP(), VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

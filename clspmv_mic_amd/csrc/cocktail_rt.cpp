@@ -1,0 +1,387 @@
+// Cocktail device runtime (src/spmv_cocktail.cpp): uploads every part of a
+// partitioned matrix, launches them into one y in the reference's order
+// (BDIA, DIA, SBELL, BELL, BCSR, SELL, ELL, CSR, COO; do_spmv :1088-1132),
+// and exposes the reference's copy helpers.
+//
+// The pass over all parts is captured once into a HIP graph, so one SpMV is
+// a single graph launch instead of up to ten kernel launches (the reference
+// pays a full launch per part, 0.27 ms each on its device, runlog.txt:65).
+// Parts accumulate into y one after another on one stream, which fixes the
+// summation order of every row: results do not depend on timing.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "clspmv_cocktail.h"
+#include "runtime.hpp"
+
+using namespace clspmv;
+
+struct clspmv_cocktail_gpu {
+    int device = 0;
+    int width = 0, height = 0, nnz = 0;
+    float* vec = nullptr;  // x, padded to a multiple of 8 floats with zeros
+    float* res = nullptr;  // y
+    int vec_len = 0;
+    std::vector<clspmv_dmat*> parts;  // launch order
+    std::vector<int> methods;
+};
+
+struct clspmv_cocktail_kernels {
+    clspmv_ctx* ctx = nullptr;
+    clspmv_cocktail_gpu* gpu = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Whether `method` can run on `A` (LDS windows and 128-bit paths have shape
+// requirements); the partitioner's choice comes from the tables and may
+// name a variant the part's shape does not allow.
+bool method_runs(const clspmv_dmat* A, int m) {
+    const int span = A->max_off - A->min_off;
+    switch (A->info.format) {
+        case CLSPMV_FMT_ELL: return m == 0 || A->hal % 4 == 0;
+        case CLSPMV_FMT_DIA:
+            if ((m & 1) && A->len % 4) return false;
+            return m < 2 || (m == 3 ? 1024 : 256) + span <= 8192;
+        case CLSPMV_FMT_BDIA: {
+            static const int rpt[5] = {1, 1, 2, 4, 4};
+            static const bool lds[5] = {false, true, true, false, true};
+            if (m == 3 && A->len % 4) return false;
+            return !lds[m] || rpt[m] * 256 + A->max_band <= 4096;
+        }
+        default: return true;
+    }
+}
+
+int launch_all(hipStream_t s, const clspmv_cocktail_gpu* g) {
+    for (size_t i = 0; i < g->parts.size(); ++i) {
+        int st = dmat_launch(s, g->parts[i], g->methods[i], g->vec, g->res);
+        if (st) return st;
+    }
+    return CLSPMV_OK;
+}
+
+int capture(clspmv_cocktail_kernels* k) {
+    hipStream_t s = k->ctx->stream;
+    CLSPMV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int st = launch_all(s, k->gpu);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (st) {
+        if (g) hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    k->graph = g;
+    CLSPMV_HIP(hipGraphInstantiate(&k->exec, g, nullptr, nullptr, 0));
+    return CLSPMV_OK;
+}
+
+template <class F>
+int add_part(clspmv_cocktail_gpu* g, clspmv_ctx* ctx, bool use, int meth, F&& upload) {
+    if (!use) return CLSPMV_OK;
+    clspmv_dmat* A = nullptr;
+    int st = upload(ctx, &A);
+    if (st) return st;
+    if (A->info.nnz == 0) {  // nothing to launch
+        clspmv_dmat_free(A);
+        return CLSPMV_OK;
+    }
+    int m = meth;
+    if (m < 0 || m >= clspmv_method_count(A->info.format) || !method_runs(A, m)) m = auto_method(A);
+    g->parts.push_back(A);
+    g->methods.push_back(m);
+    return CLSPMV_OK;
+}
+
+void destroy_gpu(clspmv_cocktail_gpu* g) {
+    if (!g) return;
+    int cur = -1;
+    hipGetDevice(&cur);
+    if (cur != g->device) (void)hipSetDevice(g->device);
+    for (clspmv_dmat* p : g->parts) clspmv_dmat_free(p);
+    if (g->vec) (void)hipFree(g->vec);
+    if (g->res) (void)hipFree(g->res);
+    delete g;
+}
+
+void destroy_kernels(clspmv_cocktail_kernels* k) {
+    if (!k) return;
+    if (k->exec) (void)hipGraphExecDestroy(k->exec);
+    if (k->graph) (void)hipGraphDestroy(k->graph);
+    clspmv_ctx_destroy(k->ctx);
+    delete k;
+}
+
+int sync(clspmv_cocktail_kernels* k) {
+    CLSPMV_HIP(hipStreamSynchronize(k->ctx->stream));
+    return CLSPMV_OK;
+}
+
+}  // namespace
+
+int clspmv_init_mat_kernels_on(int device, const clspmv_cocktail* m, const float* vec,
+                               const float* res, clspmv_cocktail_gpu** gpumat,
+                               clspmv_cocktail_kernels** kernels) {
+    if (!m || !gpumat || !kernels) return fail(CLSPMV_ERR_INVALID, "null argument");
+    *gpumat = nullptr;
+    *kernels = nullptr;
+    clspmv_cocktail_kernels* k = new clspmv_cocktail_kernels();
+    int st = clspmv_ctx_create(device, &k->ctx);
+    if (st) {
+        delete k;
+        return st;
+    }
+    clspmv_cocktail_gpu* g = new clspmv_cocktail_gpu();
+    g->device = device;
+    g->width = m->mat_width;
+    g->height = m->mat_height;
+    g->nnz = m->mat_nnz;
+    k->gpu = g;
+    auto cleanup = [&](int code) {
+        destroy_gpu(g);
+        k->gpu = nullptr;
+        destroy_kernels(k);
+        return code;
+    };
+    // x padded to a multiple of 8 (cpy_shared, src/spmv_cocktail.cpp:230).
+    g->vec_len = round_up(std::max(g->width, 1), 8);
+    hipError_t e = hipMalloc(&g->vec, sizeof(float) * g->vec_len);
+    if (e == hipSuccess) e = hipMalloc(&g->res, sizeof(float) * std::max(g->height, 1));
+    if (e == hipSuccess) e = hipMemsetAsync(g->vec, 0, sizeof(float) * g->vec_len, k->ctx->stream);
+    if (e == hipSuccess && vec && g->width)
+        e = hipMemcpyAsync(g->vec, vec, sizeof(float) * g->width, hipMemcpyHostToDevice, k->ctx->stream);
+    if (e == hipSuccess && res && g->height)
+        e = hipMemcpyAsync(g->res, res, sizeof(float) * g->height, hipMemcpyHostToDevice, k->ctx->stream);
+    else if (e == hipSuccess && g->height)
+        e = hipMemsetAsync(g->res, 0, sizeof(float) * g->height, k->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(k->ctx->stream);
+    if (e != hipSuccess)
+        return cleanup(fail(CLSPMV_ERR_ALLOC, "cocktail vectors: %s", hipGetErrorString(e)));
+
+    clspmv_ctx* c = k->ctx;
+    if ((st = add_part(g, c, m->ifusebdia, m->bdia_meth_num,
+                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_bdia(x, &m->bdia, A); })) ||
+        (st = add_part(g, c, m->ifusedia, m->dia_meth_num,
+                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_dia(x, &m->dia, A); })) ||
+        (st = add_part(g, c, m->ifusesbell, m->sbell_meth_num,
+                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_sbell(x, &m->sbell, A); })) ||
+        (st = add_part(g, c, m->ifusebell, m->bell_meth_num,
+                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_b4ell(x, &m->bell, A); })) ||
+        (st = add_part(g, c, m->ifusebcsr, m->bcsr_meth_num,
+                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_b4csr(x, &m->bcsr, A); })) ||
+        (st = add_part(g, c, m->ifusesell, m->sell_meth_num,
+                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_sell(x, &m->sell, A); })) ||
+        (st = add_part(g, c, m->ifuseell, m->ell_meth_num,
+                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_ell(x, &m->ell, A); })) ||
+        (st = add_part(g, c, m->ifusecsr, m->csr_meth_num,
+                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_csr(x, &m->csr, A); })) ||
+        (st = add_part(g, c, m->ifusecoo, m->coo_meth_num,
+                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_coo(x, &m->coo, A); })))
+        return cleanup(st);
+    if ((st = capture(k))) return cleanup(st);
+    *gpumat = g;
+    *kernels = k;
+    return CLSPMV_OK;
+}
+
+int clspmv_init_mat_kernels(const clspmv_cocktail* cpumat, const float* vec, const float* res,
+                            clspmv_cocktail_gpu** gpumat, clspmv_cocktail_kernels** kernels,
+                            int use_tex) {
+    (void)use_tex;  // no texture path on CDNA
+    return clspmv_init_mat_kernels_on(0, cpumat, vec, res, gpumat, kernels);
+}
+
+int clspmv_do_spmv_async(clspmv_cocktail_kernels* k, int ntimes) {
+    if (!k || !k->exec) return fail(CLSPMV_ERR_INVALID, "kernels not initialised");
+    int st = use_ctx_device(k->ctx);
+    if (st) return st;
+    for (int i = 0; i < ntimes; ++i) CLSPMV_HIP(hipGraphLaunch(k->exec, k->ctx->stream));
+    return CLSPMV_OK;
+}
+
+int clspmv_do_spmv(clspmv_cocktail_kernels* k, int ntimes) {
+    int st = clspmv_do_spmv_async(k, ntimes);
+    if (st) return st;
+    return sync(k);
+}
+
+namespace {
+int check_pair(clspmv_cocktail_gpu* g, clspmv_cocktail_kernels* k) {
+    if (!g || !k || k->gpu != g) return fail(CLSPMV_ERR_INVALID, "gpumat/kernels mismatch");
+    return use_ctx_device(k->ctx);
+}
+
+int copy(clspmv_cocktail_kernels* k, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    if (bytes) CLSPMV_HIP(hipMemcpyAsync(dst, src, bytes, kind, k->ctx->stream));
+    return sync(k);
+}
+}  // namespace
+
+int clspmv_cpy_vector_from_cpu(clspmv_cocktail_gpu* g, clspmv_cocktail_kernels* k, const float* vec, int n) {
+    int st = check_pair(g, k);
+    if (st) return st;
+    if (n < 0 || n > g->width) return fail(CLSPMV_ERR_INVALID, "vector size %d > width %d", n, g->width);
+    return copy(k, g->vec, vec, sizeof(float) * n, hipMemcpyHostToDevice);
+}
+
+int clspmv_cpy_vector_from_gpu(clspmv_cocktail_gpu* g, clspmv_cocktail_kernels* k, const float* d_vec, int n) {
+    int st = check_pair(g, k);
+    if (st) return st;
+    if (n < 0 || n > g->width) return fail(CLSPMV_ERR_INVALID, "vector size %d > width %d", n, g->width);
+    return copy(k, g->vec, d_vec, sizeof(float) * n, hipMemcpyDeviceToDevice);
+}
+
+int clspmv_cpy_result_from_cpu(clspmv_cocktail_gpu* g, clspmv_cocktail_kernels* k, const float* res, int n) {
+    int st = check_pair(g, k);
+    if (st) return st;
+    if (n < 0 || n > g->height) return fail(CLSPMV_ERR_INVALID, "result size %d > height %d", n, g->height);
+    return copy(k, g->res, res, sizeof(float) * n, hipMemcpyHostToDevice);
+}
+
+int clspmv_cpy_result_from_gpu(clspmv_cocktail_gpu* g, clspmv_cocktail_kernels* k, const float* d_res, int n) {
+    int st = check_pair(g, k);
+    if (st) return st;
+    if (n < 0 || n > g->height) return fail(CLSPMV_ERR_INVALID, "result size %d > height %d", n, g->height);
+    return copy(k, g->res, d_res, sizeof(float) * n, hipMemcpyDeviceToDevice);
+}
+
+int clspmv_cpy_result_to_cpu(clspmv_cocktail_gpu* g, clspmv_cocktail_kernels* k, float* res, int n) {
+    int st = check_pair(g, k);
+    if (st) return st;
+    if (n < 0 || n > g->height) return fail(CLSPMV_ERR_INVALID, "result size %d > height %d", n, g->height);
+    return copy(k, res, g->res, sizeof(float) * n, hipMemcpyDeviceToHost);
+}
+
+int clspmv_cpy_result_to_gpu(clspmv_cocktail_gpu* g, clspmv_cocktail_kernels* k, float* d_res, int n) {
+    int st = check_pair(g, k);
+    if (st) return st;
+    if (n < 0 || n > g->height) return fail(CLSPMV_ERR_INVALID, "result size %d > height %d", n, g->height);
+    return copy(k, d_res, g->res, sizeof(float) * n, hipMemcpyDeviceToDevice);
+}
+
+int clspmv_free_mat_kernels(clspmv_cocktail_gpu* g, clspmv_cocktail_kernels* k) {
+    if (k && k->ctx) {
+        use_ctx_device(k->ctx);
+        (void)hipStreamSynchronize(k->ctx->stream);
+    }
+    destroy_gpu(g);
+    if (k) k->gpu = nullptr;
+    destroy_kernels(k);
+    return CLSPMV_OK;
+}
+
+float* clspmv_cocktail_gpu_vec(clspmv_cocktail_gpu* g) { return g ? g->vec : nullptr; }
+float* clspmv_cocktail_gpu_res(clspmv_cocktail_gpu* g) { return g ? g->res : nullptr; }
+clspmv_ctx* clspmv_cocktail_kernels_ctx(clspmv_cocktail_kernels* k) { return k ? k->ctx : nullptr; }
+
+int clspmv_cocktail_profile(clspmv_cocktail_kernels* k, int ntimes, clspmv_part_timing* parts,
+                            int max_parts, int* nparts, double* graph_ms) {
+    if (!k || !k->gpu || ntimes <= 0) return fail(CLSPMV_ERR_INVALID, "bad argument");
+    int st = use_ctx_device(k->ctx);
+    if (st) return st;
+    clspmv_cocktail_gpu* g = k->gpu;
+    hipStream_t s = k->ctx->stream;
+    // Profiling accumulates into y: keep a copy and restore it afterwards.
+    float* saved = nullptr;
+    CLSPMV_HIP(hipMalloc(&saved, sizeof(float) * std::max(g->height, 1)));
+    hipError_t e = hipMemcpyAsync(saved, g->res, sizeof(float) * g->height, hipMemcpyDeviceToDevice, s);
+    int n = 0;
+    for (size_t i = 0; i < g->parts.size() && e == hipSuccess; ++i) {
+        // one warm-up launch, then ntimes timed
+        st = dmat_launch(s, g->parts[i], g->methods[i], g->vec, g->res);
+        if (!st) e = hipEventRecord(k->ctx->ev_start, s);
+        for (int r = 0; r < ntimes && !st && e == hipSuccess; ++r)
+            st = dmat_launch(s, g->parts[i], g->methods[i], g->vec, g->res);
+        if (st) break;
+        e = hipEventRecord(k->ctx->ev_stop, s);
+        if (e == hipSuccess) e = hipEventSynchronize(k->ctx->ev_stop);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, k->ctx->ev_start, k->ctx->ev_stop);
+        if (parts && n < max_parts) {
+            parts[n].format = g->parts[i]->info.format;
+            parts[n].method = g->methods[i];
+            parts[n].nnz = g->parts[i]->info.nnz;
+            parts[n].stored = g->parts[i]->info.stored;
+            parts[n].ms = (double)ms / ntimes;
+        }
+        ++n;
+    }
+    if (!st && e == hipSuccess && graph_ms) {
+        e = hipGraphLaunch(k->exec, s);
+        if (e == hipSuccess) e = hipEventRecord(k->ctx->ev_start, s);
+        for (int r = 0; r < ntimes && e == hipSuccess; ++r) e = hipGraphLaunch(k->exec, s);
+        if (e == hipSuccess) e = hipEventRecord(k->ctx->ev_stop, s);
+        if (e == hipSuccess) e = hipEventSynchronize(k->ctx->ev_stop);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, k->ctx->ev_start, k->ctx->ev_stop);
+        *graph_ms = (double)ms / ntimes;
+    }
+    hipError_t e2 = hipMemcpyAsync(g->res, saved, sizeof(float) * g->height, hipMemcpyDeviceToDevice, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(saved);
+    if (st) return st;
+    if (e != hipSuccess || e2 != hipSuccess)
+        return fail(CLSPMV_ERR_HIP, "profile failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    if (nparts) *nparts = n;
+    return CLSPMV_OK;
+}
+
+// evaluate (src/spmv_cocktail.cpp:1431) with check_and_time (:1134).
+int clspmv_evaluate(const clspmv_cocktail* cpumat, const float* vec, float* res,
+                    const float* coores, clspmv_eval_report* report) {
+    if (!cpumat || !vec || !res) return fail(CLSPMV_ERR_INVALID, "null argument");
+    clspmv_cocktail_gpu* g = nullptr;
+    clspmv_cocktail_kernels* k = nullptr;
+    int st = clspmv_init_mat_kernels(cpumat, vec, res, &g, &k, 1);
+    if (st) return st;
+    const int h = cpumat->mat_height;
+    std::vector<float> y(std::max(h, 1));
+    st = clspmv_do_spmv(k, 1);
+    if (!st) st = clspmv_cpy_result_to_cpu(g, k, y.data(), h);
+    clspmv_eval_report rep{};
+    if (!st && coores) {
+        // The reference's report (src/util.cpp:93-131): L2 distance, largest
+        // difference of magnitudes and its ratio to the larger magnitude.
+        double dist = 0, maxdiff = 0, maxratio = 0;
+        for (int i = 0; i < h; ++i) {
+            double t = (double)coores[i] - (double)y[i];
+            dist += t * t;
+            float a = std::fabs(coores[i]), b = std::fabs(y[i]);
+            double d = std::fabs((double)a - (double)b);
+            float mx = a > b ? a : b;
+            if (d > maxdiff) maxdiff = d;
+            if (mx > 0 && d / mx > maxratio) maxratio = d / mx;
+        }
+        rep.l2_distance = std::sqrt(dist);
+        rep.max_diff = maxdiff;
+        rep.max_ratio = maxratio;
+        std::printf("\nCorrectness Check: Distance %e max diff %e max diff ratio %e vec size %d\n",
+                    rep.l2_distance, rep.max_diff, rep.max_ratio, h);
+    }
+    if (!st) {
+        const int ntimes = 1000;
+        st = use_ctx_device(k->ctx);
+        hipError_t e = hipSuccess;
+        if (!st) e = hipEventRecord(k->ctx->ev_start, k->ctx->stream);
+        if (!st && e == hipSuccess) st = clspmv_do_spmv_async(k, ntimes);
+        if (!st && e == hipSuccess) e = hipEventRecord(k->ctx->ev_stop, k->ctx->stream);
+        if (!st && e == hipSuccess) e = hipEventSynchronize(k->ctx->ev_stop);
+        float ms = 0.0f;
+        if (!st && e == hipSuccess) e = hipEventElapsedTime(&ms, k->ctx->ev_start, k->ctx->ev_stop);
+        if (!st && e != hipSuccess) st = fail(CLSPMV_ERR_HIP, "evaluate timing: %s", hipGetErrorString(e));
+        if (!st) {
+            rep.ms_per_spmv = (double)ms / ntimes;
+            rep.gflops = 2.0 * cpumat->mat_nnz / (rep.ms_per_spmv * 1e-3) / 1e9;
+            std::printf("\nCocktail time %lf ms GFLOPS %lf\n\n", rep.ms_per_spmv, rep.gflops);
+        }
+    }
+    std::copy(y.begin(), y.begin() + h, res);
+    clspmv_free_mat_kernels(g, k);
+    if (report) *report = rep;
+    return st;
+}

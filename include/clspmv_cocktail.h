@@ -88,6 +88,14 @@ int clspmv_analyze_matrix(clspmv_coo_matrix* coomat, clspmv_cocktail* mat,
  * NULL disables.  Not thread safe. */
 void clspmv_set_analyze_log(void (*log)(const char* line));
 
+/* ---- autotuner (replaces src/bench_*.cpp and the benchmarking script) ----
+ * Times every kernel variant of every format on synthetic matrices
+ * generated in device memory (dimensions 1024 .. 2M rows) and writes the ten
+ * tables to <out_dir>/benchmark/*.ben in the reference's grammar, ready for
+ * clspmv_analyze_matrix.  level 0: up to 64M stored entries per probe and 8
+ * timed launches; level 1: up to 256M entries and 20 launches. */
+int clspmv_autotune(int device_ordinal, const char* out_dir, int level, int verbose);
+
 /* ---- device runtime, include/spmv_cocktail.h:166-175 ------------------- */
 typedef struct clspmv_cocktail_gpu clspmv_cocktail_gpu;
 typedef struct clspmv_cocktail_kernels clspmv_cocktail_kernels;

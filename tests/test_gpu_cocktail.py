@@ -1,0 +1,106 @@
+"""GPU parity of the Cocktail runtime: partitions from analyze_matrix on the
+synthetic table sets, run as one HIP graph per SpMV, against the FP64
+oracle of the unpartitioned matrix (tolerance 1e-5 of |y0| + |A||x|, and
+bit-exact on dyadic inputs)."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import clspmv_mic_amd as cs
+from clspmv_mic_amd import matrices
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import cases  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(HERE, "golden")
+TOL = 1e-5
+
+
+def _dyadic_copy(A):
+    v = (np.round(A.val * 4) / 64.0).astype(np.float32)
+    v[v == 0] = 1.0 / 64
+    return cs.Coo(A.row, A.col, v, A.height, A.width)
+
+
+@pytest.mark.parametrize("tset", cases.TABLE_SETS)
+@pytest.mark.parametrize("mname", cases.PARTITION_MATRICES)
+def test_cocktail_matches_oracle(oracle, mname, tset):
+    A = cases.matrix(mname)
+    ck = cs.analyze_matrix(A.copy(), os.path.join(GOLD, "ben_" + tset))
+    assert sum(ck.summary().values()) == A.nnz
+    x = matrices.uniform_vector(A.width, seed=21)
+    y0 = matrices.uniform_vector(A.height, seed=22)
+    rt = cs.CocktailRuntime(ck, x, y0)
+    rt.do_spmv(1)
+    y = rt.get_y()
+    rt.close()
+    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+    err, worst = oracle.scaled_error(y, y64, bound)
+    assert err <= TOL, f"{ck.summary()}: scaled error {err:.3e} at row {worst}"
+
+
+@pytest.mark.parametrize("tset", ["dia_first", "bdia_first", "block_first", "bell_first", "sell_first"])
+def test_cocktail_dyadic_exact(oracle, tset):
+    for mname in ["stencil27_12", "powerlaw_band", "block_thinned"]:
+        A = _dyadic_copy(cases.matrix(mname))
+        ck = cs.analyze_matrix(A.copy(), os.path.join(GOLD, "ben_" + tset))
+        x = matrices.dyadic_vector(A.width, seed=3)
+        y0 = matrices.dyadic_vector(A.height, seed=4)
+        rt = cs.CocktailRuntime(ck, x, y0)
+        rt.do_spmv(1)
+        y = rt.get_y()
+        rt.close()
+        y64, _ = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+        np.testing.assert_array_equal(y, y64.astype(np.float32), err_msg=f"{mname} {ck.summary()}")
+
+
+def test_iterated_spmv_example(oracle):
+    """The reference's example(): y = A^k x with cpy_vector_from_gpu between
+    passes (src/eval.cpp:19-72), against k FP64 SpMVs."""
+    A = matrices.laplacian_2d(48)
+    A = cs.Coo(A.row, A.col, (A.val / 8.0).astype(np.float32), A.height, A.width)
+    ck = cs.analyze_matrix(A.copy(), os.path.join(GOLD, "ben_dia_first"))
+    x = matrices.uniform_vector(A.width, seed=5)
+    rt = cs.CocktailRuntime(ck, x)
+    ref = x.astype(np.float64)
+    for _ in range(10):
+        rt.set_y(np.zeros(A.height, np.float32))
+        rt.do_spmv(1)
+        rt.y_to_x()
+        y64, _ = oracle.spmv_f64(A.row, A.col, A.val, ref.astype(np.float32), A.height)
+        ref = y64
+    y = rt.get_y()
+    rt.close()
+    np.testing.assert_allclose(y, ref, rtol=1e-4, atol=1e-6)
+
+
+def test_do_spmv_accumulates(oracle):
+    """do_spmv(k) runs k passes of y += A x."""
+    A = cases.matrix("powerlaw_20k")
+    ck = cs.analyze_matrix(A.copy(), os.path.join(GOLD, "ben_csr_first"))
+    x = matrices.uniform_vector(A.width, seed=7)
+    rt = cs.CocktailRuntime(ck, x)
+    rt.do_spmv(3)
+    y = rt.get_y()
+    rt.close()
+    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height)
+    err, _ = oracle.scaled_error(y, 3 * y64, 3 * bound)
+    assert err <= 3 * TOL
+
+
+def test_profile_restores_y(oracle):
+    A = cases.matrix("stencil27_12")
+    ck = cs.analyze_matrix(A.copy(), os.path.join(GOLD, "ben_dia_first"))
+    x = matrices.uniform_vector(A.width, seed=8)
+    rt = cs.CocktailRuntime(ck, x)
+    rt.do_spmv(1)
+    before = rt.get_y()
+    parts, gms = rt.profile(5)
+    assert parts and gms > 0
+    np.testing.assert_array_equal(rt.get_y(), before)
+    rt.close()
