@@ -59,6 +59,8 @@ hipError_t sbell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
 // ---- bandwidth probe ---------------------------------------------------------
 // dst[0..4*n4) = src[0..4*n4) with 128-bit loads/stores (16-byte aligned).
 hipError_t copy4(hipStream_t s, long long n4, const float* src, float* dst);
+// y[i] = x[hash(i) % nx] for i < n: random gathers from an nx-float vector.
+hipError_t gather(hipStream_t s, long long n, int nx, const float* x, float* y);
 
 }  // namespace launch
 }  // namespace clspmv

@@ -602,4 +602,33 @@ int clspmv_bandwidth_probe(clspmv_ctx* ctx, size_t bytes, int ntimes,
     return CLSPMV_OK;
 }
 
+int clspmv_gather_probe(clspmv_ctx* ctx, int nx, long long n, int ntimes,
+                        double* ms_per_call, double* ggathers_per_s) {
+    if (!ctx || nx <= 0 || n <= 0 || ntimes <= 0) return fail(CLSPMV_ERR_INVALID, "bad argument");
+    int st = use_ctx_device(ctx);
+    if (st) return st;
+    float *x = nullptr, *y = nullptr;
+    CLSPMV_HIP(hipMalloc(&x, sizeof(float) * nx));
+    hipError_t e = hipMalloc(&y, sizeof(float) * n);
+    if (e != hipSuccess) {
+        (void)hipFree(x);
+        return fail(CLSPMV_ERR_ALLOC, "probe buffer: %s", hipGetErrorString(e));
+    }
+    float ms = 0.0f;
+    e = hipMemsetAsync(x, 0, sizeof(float) * nx, ctx->stream);
+    if (e == hipSuccess) e = launch::gather(ctx->stream, n, nx, x, y);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_start, ctx->stream);
+    for (int i = 0; i < ntimes && e == hipSuccess; ++i) e = launch::gather(ctx->stream, n, nx, x, y);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    (void)hipFree(x);
+    (void)hipFree(y);
+    if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "gather probe failed: %s", hipGetErrorString(e));
+    const double per = (double)ms / ntimes;
+    if (ms_per_call) *ms_per_call = per;
+    if (ggathers_per_s) *ggathers_per_s = per > 0 ? (double)n / (per * 1e-3) / 1e9 : 0.0;
+    return CLSPMV_OK;
+}
+
 }  // extern "C"

@@ -117,6 +117,12 @@ int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
 int clspmv_bandwidth_probe(clspmv_ctx* ctx, size_t bytes, int ntimes,
                            double* ms_per_copy, double* gbps);
 
+/* Random-gather probe: n reads of x[hash(i) % nx] (no locality) written to
+ * a dense y.  Returns the average kernel time and billions of gathers per
+ * second: the ceiling of SpMV kernels whose column indices are random. */
+int clspmv_gather_probe(clspmv_ctx* ctx, int nx, long long n, int ntimes,
+                        double* ms_per_call, double* ggathers_per_s);
+
 /* ---- per-format drivers (include/spmv_<fmt>.h:7) ------------------------
  * spmv_<fmt>(char* oclfilename, coo_matrix<int,float>* mat, int dim2Size,
  *            int ntimes, cl_device_type) -- the reference converts the COO
