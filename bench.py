@@ -105,6 +105,7 @@ def main():
     ap.add_argument("--workload", default="powerlaw", choices=["powerlaw", "stencil27", "lap2d", "block4x4"])
     ap.add_argument("--tables", default=None, help="directory holding benchmark/*.ben")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the tables' methods and graph launches")
     ap.add_argument("--traffic", default=None, help="JSON of measured HBM bytes per launch by kernel")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -128,6 +129,12 @@ def main():
     print(f"[bench] partition {summary} ({t_analyze:.1f}s)", file=sys.stderr, flush=True)
     x = matrices.uniform_vector(A.width, seed=1)
     rt = cs.CocktailRuntime(ck, x)
+    graph = True
+    if not args.no_tune:
+        t0 = time.time()
+        graph = rt.tune(20)
+        print(f"[bench] tuned methods, launch mode {'graph' if graph else 'direct'} ({time.time() - t0:.1f}s)",
+              file=sys.stderr, flush=True)
 
     # One pass from y = 0 for the CPU cross-check, then warm-up and timing.
     rt.do_spmv(1)
@@ -178,7 +185,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc, "rows": A.height, "cols": A.width, "nnz": A.nnz,
-                   "partition": summary, "tables": os.path.relpath(tables, ROOT)},
+                   "partition": summary, "tables": os.path.relpath(tables, ROOT),
+                   "launch": "graph" if graph else "direct", "tuned": not args.no_tune},
         "hbm_gbps_algorithmic": round(total_bytes / (ms_per_step * 1e-3) / 1e9, 2),
         "roofline": {
             "bound": "hbm",

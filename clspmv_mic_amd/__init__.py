@@ -263,6 +263,8 @@ if hasattr(lib, "clspmv_init_mat_kernels"):
     _sig("clspmv_cocktail_kernels_ctx", VP, VP)
     _sig("clspmv_cocktail_profile", C.c_int, VP, C.c_int, P(PartTiming), C.c_int, P(C.c_int),
          P(C.c_double))
+    _sig("clspmv_cocktail_tune", C.c_int, VP, C.c_int, P(C.c_int))
+    _sig("clspmv_cocktail_set_graph", C.c_int, VP, C.c_int)
     _sig("clspmv_evaluate", C.c_int, P(CocktailMatrix), PF, PF, PF, P(EvalReport))
 
 
@@ -699,6 +701,17 @@ class CocktailRuntime:
         check(lib.clspmv_cocktail_profile(self.k, ntimes, parts, 16, C.byref(n), C.byref(gms)), "profile")
         return [dict(format=FORMAT_NAMES[parts[i].format], method=parts[i].method, nnz=parts[i].nnz,
                      stored=parts[i].stored, ms=parts[i].ms) for i in range(n.value)], gms.value
+
+    def tune(self, ntimes: int = 20) -> bool:
+        """Time every allowed method (and lane count) per part, keep the
+        fastest, and pick graph or plain launches; returns True for graph."""
+        g = C.c_int()
+        check(lib.clspmv_cocktail_tune(self.k, ntimes, C.byref(g)), "cocktail_tune")
+        return bool(g.value)
+
+    def set_graph(self, graph: bool):
+        """One graph launch per pass (True) or plain kernel launches."""
+        check(lib.clspmv_cocktail_set_graph(self.k, int(graph)), "cocktail_set_graph")
 
     def close(self):
         if self.k:

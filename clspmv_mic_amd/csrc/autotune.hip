@@ -136,8 +136,11 @@ int make_dia(hipStream_t s, int dim, int width, bool banded, clspmv_dmat** out) 
         m->ndiag = width;
         m->max_band = width;
         int bptr[2] = {0, width};
+        m->min_off = first;
+        m->max_off = first + width - 1;
         st = upload_array(m, &first, 1, &m->offs);
         if (!st) st = upload_array(m, bptr, 2, &m->bptr);
+        if (!st) st = upload_array(m, offs.data(), offs.size(), &m->offs_exp);
     }
     if (st) { clspmv_dmat_free(m); return st; }
     *out = m;

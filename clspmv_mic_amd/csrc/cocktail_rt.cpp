@@ -33,6 +33,7 @@ struct clspmv_cocktail_kernels {
     clspmv_cocktail_gpu* gpu = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool use_graph = true;  // false: the pass is issued as plain launches
 };
 
 namespace {
@@ -45,12 +46,12 @@ bool method_runs(const clspmv_dmat* A, int m) {
     switch (A->info.format) {
         case CLSPMV_FMT_ELL: return m == 0 || A->hal % 4 == 0;
         case CLSPMV_FMT_DIA:
-            if ((m & 1) && A->len % 4) return false;
+            if (m == 1 && A->len % 2) return false;
             return m < 2 || (m == 3 ? 1024 : 256) + span <= 8192;
         case CLSPMV_FMT_BDIA: {
-            static const int rpt[5] = {1, 1, 2, 4, 4};
+            static const int rpt[5] = {1, 1, 2, 2, 4};
             static const bool lds[5] = {false, true, true, false, true};
-            if (m == 3 && A->len % 4) return false;
+            if (m == 3 && A->len % 2) return false;
             return !lds[m] || rpt[m] * 256 + A->max_band <= 4096;
         }
         default: return true;
@@ -67,12 +68,16 @@ int launch_all(hipStream_t s, const clspmv_cocktail_gpu* g) {
 
 int capture(clspmv_cocktail_kernels* k) {
     hipStream_t s = k->ctx->stream;
+    if (k->exec) (void)hipGraphExecDestroy(k->exec);
+    if (k->graph) (void)hipGraphDestroy(k->graph);
+    k->exec = nullptr;
+    k->graph = nullptr;
     CLSPMV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int st = launch_all(s, k->gpu);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (st) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return st;
     }
     if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
@@ -101,7 +106,7 @@ int add_part(clspmv_cocktail_gpu* g, clspmv_ctx* ctx, bool use, int meth, F&& up
 void destroy_gpu(clspmv_cocktail_gpu* g) {
     if (!g) return;
     int cur = -1;
-    hipGetDevice(&cur);
+    (void)hipGetDevice(&cur);
     if (cur != g->device) (void)hipSetDevice(g->device);
     for (clspmv_dmat* p : g->parts) clspmv_dmat_free(p);
     if (g->vec) (void)hipFree(g->vec);
@@ -200,6 +205,11 @@ int clspmv_do_spmv_async(clspmv_cocktail_kernels* k, int ntimes) {
     if (!k || !k->exec) return fail(CLSPMV_ERR_INVALID, "kernels not initialised");
     int st = use_ctx_device(k->ctx);
     if (st) return st;
+    if (!k->use_graph) {
+        for (int i = 0; i < ntimes; ++i)
+            if ((st = launch_all(k->ctx->stream, k->gpu))) return st;
+        return CLSPMV_OK;
+    }
     for (int i = 0; i < ntimes; ++i) CLSPMV_HIP(hipGraphLaunch(k->exec, k->ctx->stream));
     return CLSPMV_OK;
 }
@@ -312,10 +322,11 @@ int clspmv_cocktail_profile(clspmv_cocktail_kernels* k, int ntimes, clspmv_part_
         ++n;
     }
     if (!st && e == hipSuccess && graph_ms) {
-        e = hipGraphLaunch(k->exec, s);
-        if (e == hipSuccess) e = hipEventRecord(k->ctx->ev_start, s);
-        for (int r = 0; r < ntimes && e == hipSuccess; ++r) e = hipGraphLaunch(k->exec, s);
-        if (e == hipSuccess) e = hipEventRecord(k->ctx->ev_stop, s);
+        // The whole pass in the kernels' launch mode (graph or plain).
+        st = clspmv_do_spmv_async(k, 1);
+        if (!st) e = hipEventRecord(k->ctx->ev_start, s);
+        if (!st && e == hipSuccess) st = clspmv_do_spmv_async(k, ntimes);
+        if (!st && e == hipSuccess) e = hipEventRecord(k->ctx->ev_stop, s);
         if (e == hipSuccess) e = hipEventSynchronize(k->ctx->ev_stop);
         float ms = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, k->ctx->ev_start, k->ctx->ev_stop);
@@ -328,6 +339,107 @@ int clspmv_cocktail_profile(clspmv_cocktail_kernels* k, int ntimes, clspmv_part_
     if (e != hipSuccess || e2 != hipSuccess)
         return fail(CLSPMV_ERR_HIP, "profile failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     if (nparts) *nparts = n;
+    return CLSPMV_OK;
+}
+
+namespace {
+
+// Average ms of `ntimes` runs of `fn` on the kernels' stream (one untimed
+// run first).
+template <class F>
+int time_runs(clspmv_cocktail_kernels* k, int ntimes, F&& fn, double* ms_out) {
+    hipStream_t s = k->ctx->stream;
+    int st = fn(s);
+    if (st) return st;
+    CLSPMV_HIP(hipEventRecord(k->ctx->ev_start, s));
+    for (int r = 0; r < ntimes; ++r)
+        if ((st = fn(s))) return st;
+    CLSPMV_HIP(hipEventRecord(k->ctx->ev_stop, s));
+    CLSPMV_HIP(hipEventSynchronize(k->ctx->ev_stop));
+    float ms = 0.0f;
+    CLSPMV_HIP(hipEventElapsedTime(&ms, k->ctx->ev_start, k->ctx->ev_stop));
+    *ms_out = (double)ms / ntimes;
+    return CLSPMV_OK;
+}
+
+// Lane counts worth timing around the upload's choice (powers of two).
+std::vector<int> lane_candidates(int lanes) {
+    std::vector<int> c;
+    for (int l = std::max(1, lanes / 4); l <= std::min(64, lanes * 4); l <<= 1) c.push_back(l);
+    return c;
+}
+
+int tune_parts(clspmv_cocktail_kernels* k, int ntimes, float* y) {
+    clspmv_cocktail_gpu* g = k->gpu;
+    for (size_t i = 0; i < g->parts.size(); ++i) {
+        clspmv_dmat* A = g->parts[i];
+        const int fmt = A->info.format;
+        double best = 1e30;
+        int best_m = g->methods[i], best_l = A->lanes;
+        const bool lane_fmt = fmt == CLSPMV_FMT_CSR || fmt == CLSPMV_FMT_BCSR;
+        const std::vector<int> lanes = lane_fmt ? lane_candidates(A->lanes) : std::vector<int>{A->lanes};
+        const int upload_lanes = A->lanes;
+        for (int m = 0; m < clspmv_method_count(fmt); ++m) {
+            if (!method_runs(A, m)) continue;
+            const bool uses_lanes = (fmt == CLSPMV_FMT_CSR && m == 0) || fmt == CLSPMV_FMT_BCSR;
+            for (int l : uses_lanes ? lanes : std::vector<int>{upload_lanes}) {
+                A->lanes = l;
+                double ms = 0.0;
+                int st = time_runs(k, ntimes, [&](hipStream_t s) { return dmat_launch(s, A, m, g->vec, y); }, &ms);
+                if (st) {
+                    A->lanes = upload_lanes;
+                    return st;
+                }
+                if (ms < best) {
+                    best = ms;
+                    best_m = m;
+                    best_l = l;
+                }
+            }
+        }
+        g->methods[i] = best_m;
+        A->lanes = best_l;
+    }
+    return CLSPMV_OK;
+}
+
+}  // namespace
+
+int clspmv_cocktail_tune(clspmv_cocktail_kernels* k, int ntimes, int* graph) {
+    if (!k || !k->gpu || ntimes <= 0) return fail(CLSPMV_ERR_INVALID, "bad argument");
+    int st = use_ctx_device(k->ctx);
+    if (st) return st;
+    clspmv_cocktail_gpu* g = k->gpu;
+    hipStream_t s = k->ctx->stream;
+    // Tuning launches accumulate into y: keep a copy and restore it after.
+    float* saved = nullptr;
+    const size_t ybytes = sizeof(float) * g->height;
+    CLSPMV_HIP(hipMalloc(&saved, sizeof(float) * std::max(g->height, 1)));
+    hipError_t e = hipMemcpyAsync(saved, g->res, ybytes, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) st = fail(CLSPMV_ERR_HIP, "tune: %s", hipGetErrorString(e));
+    if (!st) st = tune_parts(k, ntimes, g->res);
+    if (!st) st = capture(k);
+    double t_graph = 0.0, t_direct = 0.0;
+    if (!st)
+        st = time_runs(k, ntimes, [&](hipStream_t q) -> int {
+            CLSPMV_HIP(hipGraphLaunch(k->exec, q));
+            return CLSPMV_OK;
+        }, &t_graph);
+    if (!st) st = time_runs(k, ntimes, [&](hipStream_t q) { return launch_all(q, g); }, &t_direct);
+    e = hipMemcpyAsync(g->res, saved, ybytes, hipMemcpyDeviceToDevice, s);
+    hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(saved);
+    if (st) return st;
+    if (e != hipSuccess || e2 != hipSuccess)
+        return fail(CLSPMV_ERR_HIP, "tune: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    k->use_graph = t_graph <= t_direct;
+    if (graph) *graph = k->use_graph ? 1 : 0;
+    return CLSPMV_OK;
+}
+
+int clspmv_cocktail_set_graph(clspmv_cocktail_kernels* k, int graph) {
+    if (!k || !k->exec) return fail(CLSPMV_ERR_INVALID, "kernels not initialised");
+    k->use_graph = graph != 0;
     return CLSPMV_OK;
 }
 

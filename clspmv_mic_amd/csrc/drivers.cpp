@@ -34,8 +34,8 @@ struct DriverRun {
     std::vector<float> first;
     ~DriverRun() {
         if (ctx) {
-            if (x) hipFree(x);
-            if (y) hipFree(y);
+            if (x) (void)hipFree(x);
+            if (y) (void)hipFree(y);
             clspmv_ctx_destroy(ctx);
         }
     }
@@ -249,7 +249,7 @@ int clspmv_spmv_bdia(const char*, clspmv_coo_matrix* mat, int, int ntimes, int d
                      return st;
                  },
                  [](int meth, const clspmv_dmat* A) {
-                     static const int rpt[5] = {1, 1, 2, 4, 4};
+                     static const int rpt[5] = {1, 1, 2, 2, 4};
                      static const bool lds[5] = {false, true, true, false, true};
                      return !lds[meth] || rpt[meth] * 256 + A->max_band <= 4096;
                  });

@@ -9,6 +9,8 @@
 //   DIA : one window covering [min offset, max offset + rows of the group)
 //   BDIA: one window per band (consecutive offsets), double buffered so that
 //         staging band b+1 needs a single barrier.
+// BDIA methods without LDS run the DIA kernels over the band's diagonals
+// listed one by one (the runtime expands the band offsets at upload).
 // Out-of-range columns read zero, which replaces the reference's padded
 // `vec_extended` copy of x (src/spmv_cocktail.cpp:253-301).  Unlike the
 // reference's gpu_bdia there is no band-count or band-width limit
@@ -29,42 +31,80 @@ __device__ __forceinline__ float x_at(const float* __restrict__ x, int c, int co
 
 // ---- DIA ---------------------------------------------------------------------
 
+// One row per lane; U diagonals per iteration with independent
+// accumulators so that U data streams and U x reads are in flight per lane.
+template <int U>
 __global__ __launch_bounds__(kBlock) void dia1_kernel(
     int rows, int cols, int len, int ndiag, const int* __restrict__ off,
     const float* __restrict__ data, const float* __restrict__ x,
     float* __restrict__ y) {
     const int r = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (r >= rows) return;
-    float a0 = 0.0f, a1 = 0.0f;
+    float a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = 0.0f;
     int j = 0;
-    for (; j + 1 < ndiag; j += 2) {
-        a0 = fmaf(ld_stream(data + (size_t)j * len + r), x_at(x, r + off[j], cols), a0);
-        a1 = fmaf(ld_stream(data + (size_t)(j + 1) * len + r), x_at(x, r + off[j + 1], cols), a1);
+    for (; j + U <= ndiag; j += U) {
+        float d[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) d[u] = ld_stream(data + (size_t)(j + u) * len + r);
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x_at(x, r + off[j + u], cols);
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = fmaf(d[u], xv[u], a[u]);
     }
-    if (j < ndiag) a0 = fmaf(ld_stream(data + (size_t)j * len + r), x_at(x, r + off[j], cols), a0);
-    y[r] += a0 + a1;
+    for (; j < ndiag; ++j) a[0] = fmaf(ld_stream(data + (size_t)j * len + r), x_at(x, r + off[j], cols), a[0]);
+    float sum = 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) sum += a[u];
+    y[r] += sum;
 }
 
-// Four consecutive rows per lane, 128-bit data loads (len % 4 == 0).
-__global__ __launch_bounds__(kBlock) void dia4_kernel(
+// Two consecutive rows per lane (64-bit data loads, len % 2 == 0), U
+// diagonals per iteration.
+template <int U>
+__global__ __launch_bounds__(kBlock) void dia2_kernel(
     int rows, int cols, int len, int ndiag, const int* __restrict__ off,
     const float* __restrict__ data, const float* __restrict__ x,
     float* __restrict__ y) {
-    const int r = (xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) * 4;
+    const int r = (xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) * 2;
     if (r >= rows) return;
-    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int j = 0; j < ndiag; ++j) {
-        const float4 d = ld_stream(reinterpret_cast<const float4*>(data + (size_t)j * len + r));
-        const int c = r + off[j];
-        a.x = fmaf(d.x, x_at(x, c + 0, cols), a.x);
-        a.y = fmaf(d.y, x_at(x, c + 1, cols), a.y);
-        a.z = fmaf(d.z, x_at(x, c + 2, cols), a.z);
-        a.w = fmaf(d.w, x_at(x, c + 3, cols), a.w);
+    float a0[U], a1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a0[u] = a1[u] = 0.0f;
+    int j = 0;
+    for (; j + U <= ndiag; j += U) {
+        float2 d[U];
+        float x0[U], x1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            d[u] = *reinterpret_cast<const float2*>(data + (size_t)(j + u) * len + r);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = r + off[j + u];
+            x0[u] = x_at(x, c, cols);
+            x1[u] = x_at(x, c + 1, cols);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a0[u] = fmaf(d[u].x, x0[u], a0[u]);
+            a1[u] = fmaf(d[u].y, x1[u], a1[u]);
+        }
     }
-    y[r] += a.x;
-    if (r + 1 < rows) y[r + 1] += a.y;
-    if (r + 2 < rows) y[r + 2] += a.z;
-    if (r + 3 < rows) y[r + 3] += a.w;
+    for (; j < ndiag; ++j) {
+        const float2 d = *reinterpret_cast<const float2*>(data + (size_t)j * len + r);
+        const int c = r + off[j];
+        a0[0] = fmaf(d.x, x_at(x, c, cols), a0[0]);
+        a1[0] = fmaf(d.y, x_at(x, c + 1, cols), a1[0]);
+    }
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        s0 += a0[u];
+        s1 += a1[u];
+    }
+    y[r] += s0;
+    if (r + 1 < rows) y[r + 1] += s1;
 }
 
 // RPT rows per lane (strided by the work-group size) with the work-group's x
@@ -101,40 +141,6 @@ __global__ __launch_bounds__(kBlock) void dia_lds_kernel(
 }
 
 // ---- BDIA --------------------------------------------------------------------
-
-template <int RPT>
-__global__ __launch_bounds__(kBlock) void bdia_direct_kernel(
-    int rows, int cols, int len, int nband, const int* __restrict__ boff,
-    const int* __restrict__ bptr, const float* __restrict__ data,
-    const float* __restrict__ x, float* __restrict__ y) {
-    // RPT == 1: one row per lane; RPT == 4: four consecutive rows per lane
-    // with 128-bit data loads (len % 4 == 0).
-    const int r = (xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) * RPT;
-    if (r >= rows) return;
-    float a[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) a[q] = 0.0f;
-    for (int b = 0; b < nband; ++b) {
-        const int d0 = bptr[b], d1 = bptr[b + 1];
-        const int o = boff[b] - d0;  // offset of diagonal d is o + d
-        for (int d = d0; d < d1; ++d) {
-            const float* dp = data + (size_t)d * len + r;
-            const int c = r + o + d;
-            if (RPT == 4) {
-                const float4 v = ld_stream(reinterpret_cast<const float4*>(dp));
-                a[0] = fmaf(v.x, x_at(x, c + 0, cols), a[0]);
-                a[1 % RPT] = fmaf(v.y, x_at(x, c + 1, cols), a[1 % RPT]);
-                a[2 % RPT] = fmaf(v.z, x_at(x, c + 2, cols), a[2 % RPT]);
-                a[3 % RPT] = fmaf(v.w, x_at(x, c + 3, cols), a[3 % RPT]);
-            } else {
-                a[0] = fmaf(ld_stream(dp), x_at(x, c, cols), a[0]);
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < RPT; ++q)
-        if (r + q < rows) y[r + q] += a[q];
-}
 
 // RPT rows per lane strided by the work-group size; per band an LDS window
 // [r0 + first offset, r0 + first offset + RPT*kBlock + width - 1), double
@@ -188,12 +194,12 @@ hipError_t dia(hipStream_t s, int rows, int cols, int len, int ndiag,
     const int span = max_off - min_off;
     switch (method) {
         case 0:
-            hipLaunchKernelGGL(dia1_kernel, dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s,
+            hipLaunchKernelGGL(dia1_kernel<8>, dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s,
                                rows, cols, len, ndiag, offsets, data, x, y);
             break;
         case 1:
-            if (len % 4) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(dia4_kernel, dim3(ceil_div(rows, 4 * kBlock)), dim3(kBlock), 0, s,
+            if (len % 2) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(dia2_kernel<4>, dim3(ceil_div(rows, 2 * kBlock)), dim3(kBlock), 0, s,
                                rows, cols, len, ndiag, offsets, data, x, y);
             break;
         case 2:
@@ -218,10 +224,6 @@ hipError_t bdia(hipStream_t s, int rows, int cols, int len, int nband,
     if (rows <= 0 || nband <= 0) return hipSuccess;
     const int half = kLdsWindow / 2;
     switch (method) {
-        case 0:
-            hipLaunchKernelGGL(bdia_direct_kernel<1>, dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s,
-                               rows, cols, len, nband, band_off, bptr, data, x, y);
-            break;
         case 1:
             if (kBlock + max_band_width > half) return hipErrorInvalidValue;
             hipLaunchKernelGGL(bdia_lds_kernel<1>, dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s,
@@ -230,11 +232,6 @@ hipError_t bdia(hipStream_t s, int rows, int cols, int len, int nband,
         case 2:
             if (2 * kBlock + max_band_width > half) return hipErrorInvalidValue;
             hipLaunchKernelGGL(bdia_lds_kernel<2>, dim3(ceil_div(rows, 2 * kBlock)), dim3(kBlock), 0, s,
-                               rows, cols, len, nband, band_off, bptr, data, x, y);
-            break;
-        case 3:
-            if (len % 4) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(bdia_direct_kernel<4>, dim3(ceil_div(rows, 4 * kBlock)), dim3(kBlock), 0, s,
                                rows, cols, len, nband, band_off, bptr, data, x, y);
             break;
         case 4:

@@ -36,11 +36,15 @@ hipError_t sell(hipStream_t s, int rows, int slice_height, const int* ptr,
 
 // ---- DIA / BDIA -------------------------------------------------------------
 // rows = number of rows computed (min(height, dia_length_aligned)).
+// Methods: 0 one row per lane, 1 two rows per lane (len % 2 == 0),
+// 2 / 3 one / four rows per lane with an LDS x window.
 hipError_t dia(hipStream_t s, int rows, int cols, int len, int ndiag,
                const int* offsets, int min_off, int max_off, int method,
                const float* data, const float* x, float* y);
 // Largest x window (floats) a work-group stages in LDS.
 constexpr int kLdsWindow = 8192;
+// LDS-window methods 1, 2, 4 only; methods 0 and 3 are run by dia() over
+// the expanded diagonal offsets.
 hipError_t bdia(hipStream_t s, int rows, int cols, int len, int nband,
                 const int* band_off, const int* bptr, int max_band_width,
                 int method, const float* data, const float* x, float* y);

@@ -54,7 +54,7 @@ clspmv_dmat* new_dmat(int format, const clspmv_matinfo& mi) {
     m->info.nnz = mi.nnz;
     m->info.bheight = 1;
     m->info.bwidth = 1;
-    hipGetDevice(&m->device);
+    (void)hipGetDevice(&m->device);
     return m;
 }
 
@@ -89,10 +89,8 @@ int auto_method(const clspmv_dmat* A) {
             return A->max_row > 8.0 * std::max(avg, 4.0) ? 1 : 0;
         }
         case CLSPMV_FMT_ELL: return A->hal % 4 == 0 ? 1 : 0;
-        case CLSPMV_FMT_DIA: return A->len % 4 == 0 ? 1 : 0;
-        case CLSPMV_FMT_BDIA:
-            if (4 * 256 + A->max_band <= launch::kLdsWindow / 2) return 4;
-            return A->len % 4 == 0 ? 3 : 0;
+        case CLSPMV_FMT_DIA: return 0;
+        case CLSPMV_FMT_BDIA: return 0;
         default: return 0;
     }
 }
@@ -126,8 +124,8 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
             break;
         case CLSPMV_FMT_DIA: {
             const int span = A->max_off - A->min_off;
-            if ((method & 1) && A->len % 4)
-                return fail(CLSPMV_ERR_UNSUPPORTED, "DIA method %d needs aligned length %% 4 == 0", method);
+            if (method == 1 && A->len % 2)
+                return fail(CLSPMV_ERR_UNSUPPORTED, "DIA method 1 needs an even aligned length");
             if (method >= 2 && (method == 3 ? 4 : 1) * 256 + span > launch::kLdsWindow)
                 return fail(CLSPMV_ERR_UNSUPPORTED, "DIA method %d: offset span %d too wide for LDS", method, span);
             e = launch::dia(s, std::min(I.rows, A->len), I.cols, A->len, A->ndiag, A->offs,
@@ -135,12 +133,17 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
             break;
         }
         case CLSPMV_FMT_BDIA: {
-            static const int rpt[5] = {1, 1, 2, 4, 4};
+            static const int rpt[5] = {1, 1, 2, 2, 4};
             static const bool lds[5] = {false, true, true, false, true};
             if (lds[method] && rpt[method] * 256 + A->max_band > launch::kLdsWindow / 2)
                 return fail(CLSPMV_ERR_UNSUPPORTED, "BDIA method %d: band width %d too wide for LDS", method, A->max_band);
-            if (method == 3 && A->len % 4)
-                return fail(CLSPMV_ERR_UNSUPPORTED, "BDIA method 3 needs aligned length %% 4 == 0");
+            if (method == 0 || method == 3) {
+                if (method == 3 && A->len % 2)
+                    return fail(CLSPMV_ERR_UNSUPPORTED, "BDIA method 3 needs an even aligned length");
+                e = launch::dia(s, std::min(I.rows, A->len), I.cols, A->len, A->ndiag, A->offs_exp,
+                                A->min_off, A->max_off, method == 0 ? 0 : 1, A->val, x, y);
+                break;
+            }
             e = launch::bdia(s, std::min(I.rows, A->len), I.cols, A->len, A->nband, A->offs,
                              A->bptr, A->max_band, method, A->val, x, y);
             break;
@@ -195,10 +198,10 @@ int clspmv_ctx_create(int device_ordinal, clspmv_ctx** out) {
 int clspmv_ctx_destroy(clspmv_ctx* ctx) {
     if (!ctx) return CLSPMV_OK;
     use_ctx_device(ctx);
-    hipStreamSynchronize(ctx->stream);
-    hipEventDestroy(ctx->ev_start);
-    hipEventDestroy(ctx->ev_stop);
-    hipStreamDestroy(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(ctx->ev_start);
+    (void)hipEventDestroy(ctx->ev_stop);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CLSPMV_OK;
 }
@@ -427,6 +430,15 @@ int clspmv_upload_bdia(clspmv_ctx* ctx, const clspmv_bdia_matrix* m, clspmv_dmat
     }
     const int ndiag = nb > 0 ? m->bdia_bptr[nb] : 0;
     d->ndiag = ndiag;
+    std::vector<int> exp_off;
+    for (int b = 0; b < nb; ++b)
+        for (int j = m->bdia_bptr[b]; j < m->bdia_bptr[b + 1]; ++j)
+            exp_off.push_back(m->bdia_offsets[b] + (j - m->bdia_bptr[b]));
+    if (!exp_off.empty()) {
+        d->min_off = *std::min_element(exp_off.begin(), exp_off.end());
+        d->max_off = *std::max_element(exp_off.begin(), exp_off.end());
+    }
+    UPLOAD_CHECK(upload_array(d, exp_off.data(), exp_off.size(), &d->offs_exp));
     d->info.stored = (long long)ndiag * m->bdia_length_aligned;
     UPLOAD_CHECK(upload_array(d, m->bdia_offsets, (size_t)nb, &d->offs));
     UPLOAD_CHECK(upload_array(d, m->bdia_bptr, (size_t)nb + 1, &d->bptr));
@@ -524,10 +536,10 @@ int clspmv_upload_sbell(clspmv_ctx* ctx, const clspmv_sbell_matrix* m, clspmv_dm
 int clspmv_dmat_free(clspmv_dmat* m) {
     if (!m) return CLSPMV_OK;
     int cur = -1;
-    hipGetDevice(&cur);
-    if (cur != m->device) hipSetDevice(m->device);
-    for (void* p : m->bufs) hipFree(p);
-    if (cur != m->device && cur >= 0) hipSetDevice(cur);
+    (void)hipGetDevice(&cur);
+    if (cur != m->device) (void)hipSetDevice(m->device);
+    for (void* p : m->bufs) (void)hipFree(p);
+    if (cur != m->device && cur >= 0) (void)hipSetDevice(cur);
     delete m;
     return CLSPMV_OK;
 }
@@ -582,7 +594,7 @@ int clspmv_bandwidth_probe(clspmv_ctx* ctx, size_t bytes, int ntimes,
     CLSPMV_HIP(hipMalloc(&a, n4 * 16));
     hipError_t e = hipMalloc(&b, n4 * 16);
     if (e != hipSuccess) {
-        hipFree(a);
+        (void)hipFree(a);
         return fail(CLSPMV_ERR_ALLOC, "probe buffer: %s", hipGetErrorString(e));
     }
     float ms = 0.0f;
@@ -593,8 +605,8 @@ int clspmv_bandwidth_probe(clspmv_ctx* ctx, size_t bytes, int ntimes,
     if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
-    hipFree(a);
-    hipFree(b);
+    (void)hipFree(a);
+    (void)hipFree(b);
     if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "probe failed: %s", hipGetErrorString(e));
     const double per = (double)ms / ntimes;
     if (ms_per_copy) *ms_per_copy = per;

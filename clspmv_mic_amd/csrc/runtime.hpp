@@ -38,6 +38,7 @@ struct clspmv_dmat {
     // DIA / BDIA: offsets, band pointers, aligned length.
     int* offs = nullptr;
     int* bptr = nullptr;
+    int* offs_exp = nullptr;  // BDIA: one offset per diagonal
     int len = 0, ndiag = 0, nband = 0, min_off = 0, max_off = 0, max_band = 0;
     // Block formats: block rows, aligned sizes, group lanes.
     int brows = 0, aligned = 0, f4 = 0;

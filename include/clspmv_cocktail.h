@@ -91,7 +91,7 @@ void clspmv_set_analyze_log(void (*log)(const char* line));
 /* ---- autotuner (replaces src/bench_*.cpp and the benchmarking script) ----
  * Times every kernel variant of every format on synthetic matrices
  * generated in device memory (dimensions 1024 .. 2M rows) and writes the ten
- * tables to <out_dir>/benchmark/*.ben in the reference's grammar, ready for
+ * tables to out_dir/benchmark/NAME.ben in the reference's grammar, ready for
  * clspmv_analyze_matrix.  level 0: up to 64M stored entries per probe and 8
  * timed launches; level 1: up to 256M entries and 20 launches. */
 int clspmv_autotune(int device_ordinal, const char* out_dir, int level, int verbose);
@@ -151,6 +151,21 @@ typedef struct {
 int clspmv_cocktail_profile(clspmv_cocktail_kernels* kernels, int ntimes,
                             clspmv_part_timing* parts, int max_parts,
                             int* nparts, double* graph_ms);
+
+/* Measured launch tuning, the device-side counterpart of the reference's
+ * runtime variant probe (init_mat_kernels with use_tex,
+ * src/spmv_cocktail.cpp:1590): every method a part's shape allows (and,
+ * for CSR method 0 and BCSR, the neighbouring lane counts) is timed with
+ * HIP events over ntimes launches and the fastest kept; then the captured
+ * graph is re-built and compared with plain stream launches of the same
+ * pass, and the faster launch mode is kept.  y is preserved.  Tuning picks
+ * by timing, so two processes may settle on different methods whose
+ * results differ by rounding; untuned kernels follow the tables' choice.
+ * `graph` (may be null) receives 1 when passes run as a graph launch. */
+int clspmv_cocktail_tune(clspmv_cocktail_kernels* kernels, int ntimes, int* graph);
+/* Sets the launch mode by hand: 1 = one graph launch per pass (the default
+ * after init), 0 = plain launches of the parts' kernels. */
+int clspmv_cocktail_set_graph(clspmv_cocktail_kernels* kernels, int graph);
 
 /* evaluate (src/spmv_cocktail.cpp:1431): runs one SpMV from res, compares
  * with coores (the caller's expected y, e.g. from a serial COO SpMV) using

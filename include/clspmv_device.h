@@ -61,10 +61,10 @@ enum clspmv_format {
  *   COO  0: one-pass segmented reduction + carry fix-up
  *   ELL  0: one row per lane          1: four rows per lane (128-bit loads)
  *   SELL 0/1: one row per lane (slice height is a property of the data)
- *   DIA  0: row per lane              1: four rows per lane
+ *   DIA  0: row per lane              1: two rows per lane (64-bit loads)
  *        2: row per lane, x window in LDS   3: four rows, x window in LDS
  *   BDIA 0: row per lane              1: row per lane, LDS x window per band
- *        2: two rows per lane, LDS    3: four rows per lane
+ *        2: two rows per lane, LDS    3: two rows per lane (64-bit loads)
  *        4: four rows per lane, LDS x window per band
  *   BCSR 0: lane group per block row (group size from blocks per row)
  *   BELL 0: lane per block row, scalar FMA   1: float4 accumulators

@@ -104,3 +104,24 @@ def test_profile_restores_y(oracle):
     assert parts and gms > 0
     np.testing.assert_array_equal(rt.get_y(), before)
     rt.close()
+
+
+@pytest.mark.parametrize("mname", ["stencil27_12", "powerlaw_band", "block_thinned"])
+def test_tuned_runtime_dyadic_exact(oracle, mname):
+    """Measured tuning may pick any allowed method or lane count per part and
+    either launch mode; y is kept and every choice stays exact on dyadic
+    inputs."""
+    A = _dyadic_copy(cases.matrix(mname))
+    ck = cs.analyze_matrix(A.copy(), cs.TABLES_DIR)
+    x = matrices.dyadic_vector(A.width, seed=5)
+    y0 = matrices.dyadic_vector(A.height, seed=6)
+    rt = cs.CocktailRuntime(ck, x, y0)
+    rt.tune(3)
+    np.testing.assert_array_equal(rt.get_y(), y0)
+    y64, _ = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+    for graph in (True, False):
+        rt.set_y(y0)
+        rt.set_graph(graph)
+        rt.do_spmv(1)
+        np.testing.assert_array_equal(rt.get_y(), y64.astype(np.float32), err_msg=f"graph={graph}")
+    rt.close()
