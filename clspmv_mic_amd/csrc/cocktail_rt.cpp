@@ -461,10 +461,11 @@ int clspmv_evaluate(const clspmv_cocktail* cpumat, const float* vec, float* res,
         // difference of magnitudes and its ratio to the larger magnitude.
         double dist = 0, maxdiff = 0, maxratio = 0;
         for (int i = 0; i < h; ++i) {
-            double t = (double)coores[i] - (double)y[i];
+            // Differences in float, then widened (src/util.cpp:79,108).
+            double t = (double)(float)(coores[i] - y[i]);
             dist += t * t;
             float a = std::fabs(coores[i]), b = std::fabs(y[i]);
-            double d = std::fabs((double)a - (double)b);
+            double d = std::fabs((double)(float)(a - b));
             float mx = a > b ? a : b;
             if (d > maxdiff) maxdiff = d;
             if (mx > 0 && d / mx > maxratio) maxratio = d / mx;

@@ -244,11 +244,13 @@ void oracle_two_vec_compare(const float* a, const float* b, int n, double* out) 
     double dist = 0.0, maxdiff = 0.0, maxratio = 0.0;
     int maxid = 0, count = 0;
     for (int i = 0; i < n; ++i) {
-        double t = (double)a[i] - (double)b[i];
+        /* Both differences are taken in float and then widened, as the
+         * reference does (distance :79, two_vec_compare :108). */
+        double t = (double)(float)(a[i] - b[i]);
         dist += t * t;
         float ta = a[i] < 0 ? -a[i] : a[i];
         float tb = b[i] < 0 ? -b[i] : b[i];
-        double diff = (double)ta - (double)tb;
+        double diff = (double)(float)(ta - tb);
         if (diff < 0) diff = -diff;
         float m = ta > tb ? ta : tb;
         double ratio = m > 0 ? diff / m : 0.0;
