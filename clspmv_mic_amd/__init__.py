@@ -151,7 +151,8 @@ class CocktailMatrix(C.Structure):
 class DmatInfo(C.Structure):
     _fields_ = [("format", C.c_int), ("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int),
                 ("stored", C.c_longlong), ("device_bytes", C.c_size_t), ("bheight", C.c_int),
-                ("bwidth", C.c_int), ("slice_height", C.c_int)]
+                ("bwidth", C.c_int), ("slice_height", C.c_int), ("panel_width", C.c_int),
+                ("panel_active", C.c_int), ("plain", C.c_int)]
 
 
 class DriverReport(C.Structure):
@@ -232,6 +233,9 @@ for _n, _T in [("csr", CsrMatrix), ("coo", CooMatrix), ("ell", EllMatrix), ("sel
     _sig(f"clspmv_upload_{_n}", C.c_int, VP, P(_T), P(VP))
 _sig("clspmv_dmat_free", C.c_int, VP)
 _sig("clspmv_dmat_get_info", C.c_int, VP, P(DmatInfo))
+_sig("clspmv_upload_coo_ex", C.c_int, VP, P(CooMatrix), C.c_int, P(VP))
+_sig("clspmv_upload_csr_ex", C.c_int, VP, P(CsrMatrix), C.c_int, P(VP))
+_sig("clspmv_dmat_set_panel", C.c_int, VP, C.c_int)
 _sig("clspmv_spmv", C.c_int, VP, VP, C.c_int, VP, VP)
 _sig("clspmv_time_spmv", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int, C.c_int, P(C.c_double))
 _sig("clspmv_bandwidth_probe", C.c_int, VP, C.c_size_t, C.c_int, P(C.c_double), P(C.c_double))
@@ -547,9 +551,17 @@ class DeviceVector:
     def zero(self):
         check(lib.clspmv_memset_zero(self.ctx.h, self.p, self.n * 4), "memset")
 
+    def view(self, offset: int) -> "DeviceVector":
+        """The floats from `offset` on, sharing this buffer (not owned)."""
+        v = DeviceVector.__new__(DeviceVector)
+        v.ctx, v.n, v._base = self.ctx, self.n - int(offset), self
+        v.p = VP((self.p.value or 0) + 4 * int(offset))
+        v._owned = False
+        return v
+
     def __del__(self):
         try:
-            if self.p:
+            if self.p and getattr(self, "_owned", True):
                 lib.clspmv_device_free(self.ctx.h, self.p)
         except Exception:
             pass
@@ -560,17 +572,37 @@ _UPLOAD = {CsrMatrix: "csr", CooMatrix: "coo", EllMatrix: "ell", SellMatrix: "se
            SbellMatrix: "sbell"}
 
 
-class DeviceMatrix:
-    """A device-resident matrix of any storage format."""
+PANEL_AUTO = -1
+PANEL_BOTH = -2
 
-    def __init__(self, ctx: Context, m):
+
+class DeviceMatrix:
+    """A device-resident matrix of any storage format.  For COO and CSR,
+    ``panel`` selects the device encoding (clspmv_upload_{coo,csr}_ex):
+    0 own arrays, a width, PANEL_AUTO or PANEL_BOTH; None is the plain
+    upload call."""
+
+    def __init__(self, ctx: Context, m, panel: Optional[int] = None):
         s = m.s if isinstance(m, Owned) else (m.c if isinstance(m, Coo) else m)
         h = VP()
-        check(getattr(lib, f"clspmv_upload_{_UPLOAD[type(s)]}")(ctx.h, C.byref(s), C.byref(h)), "upload")
+        name = _UPLOAD[type(s)]
+        if panel is not None:
+            if name not in ("coo", "csr"):
+                raise ValueError("panel encodings exist for COO and CSR only")
+            check(getattr(lib, f"clspmv_upload_{name}_ex")(ctx.h, C.byref(s), int(panel), C.byref(h)),
+                  "upload")
+        else:
+            check(getattr(lib, f"clspmv_upload_{name}")(ctx.h, C.byref(s), C.byref(h)), "upload")
         self.ctx, self.h = ctx, h
-        info = DmatInfo()
-        check(lib.clspmv_dmat_get_info(h, C.byref(info)), "dmat_info")
-        self.info = info
+        self.info = DmatInfo()
+        self._refresh()
+
+    def _refresh(self):
+        check(lib.clspmv_dmat_get_info(self.h, C.byref(self.info)), "dmat_info")
+
+    def set_panel(self, use: bool):
+        check(lib.clspmv_dmat_set_panel(self.h, int(use)), "dmat_set_panel")
+        self._refresh()
 
     @property
     def format(self) -> int:

@@ -184,9 +184,13 @@ int clspmv_init_mat_kernels_on(int device, const clspmv_cocktail* m, const float
         (st = add_part(g, c, m->ifuseell, m->ell_meth_num,
                        [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_ell(x, &m->ell, A); })) ||
         (st = add_part(g, c, m->ifusecsr, m->csr_meth_num,
-                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_csr(x, &m->csr, A); })) ||
+                       [&](clspmv_ctx* x, clspmv_dmat** A) {
+                           return clspmv_upload_csr_ex(x, &m->csr, CLSPMV_PANEL_BOTH, A);
+                       })) ||
         (st = add_part(g, c, m->ifusecoo, m->coo_meth_num,
-                       [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_coo(x, &m->coo, A); })))
+                       [&](clspmv_ctx* x, clspmv_dmat** A) {
+                           return clspmv_upload_coo_ex(x, &m->coo, CLSPMV_PANEL_BOTH, A);
+                       })))
         return cleanup(st);
     if ((st = capture(k))) return cleanup(st);
     *gpumat = g;
@@ -376,29 +380,43 @@ int tune_parts(clspmv_cocktail_kernels* k, int ntimes, float* y) {
         const int fmt = A->info.format;
         double best = 1e30;
         int best_m = g->methods[i], best_l = A->lanes;
+        bool best_panel = A->use_panel;
         const bool lane_fmt = fmt == CLSPMV_FMT_CSR || fmt == CLSPMV_FMT_BCSR;
         const std::vector<int> lanes = lane_fmt ? lane_candidates(A->lanes) : std::vector<int>{A->lanes};
         const int upload_lanes = A->lanes;
-        for (int m = 0; m < clspmv_method_count(fmt); ++m) {
+        auto trial = [&](int m, int l, bool panel) -> int {
+            A->lanes = l;
+            A->use_panel = panel;
+            double ms = 0.0;
+            int st = time_runs(k, ntimes, [&](hipStream_t s) { return dmat_launch(s, A, m, g->vec, y); }, &ms);
+            if (st) return st;
+            if (ms < best) {
+                best = ms;
+                best_m = m;
+                best_l = l;
+                best_panel = panel;
+            }
+            return CLSPMV_OK;
+        };
+        int st = CLSPMV_OK;
+        if (A->pw)  // the panel encoding runs the same kernels for every method
+            st = trial(0, upload_lanes, true);
+        for (int m = 0; !st && A->plain && m < clspmv_method_count(fmt); ++m) {
             if (!method_runs(A, m)) continue;
             const bool uses_lanes = (fmt == CLSPMV_FMT_CSR && m == 0) || fmt == CLSPMV_FMT_BCSR;
-            for (int l : uses_lanes ? lanes : std::vector<int>{upload_lanes}) {
-                A->lanes = l;
-                double ms = 0.0;
-                int st = time_runs(k, ntimes, [&](hipStream_t s) { return dmat_launch(s, A, m, g->vec, y); }, &ms);
-                if (st) {
-                    A->lanes = upload_lanes;
-                    return st;
-                }
-                if (ms < best) {
-                    best = ms;
-                    best_m = m;
-                    best_l = l;
-                }
-            }
+            for (int l : uses_lanes ? lanes : std::vector<int>{upload_lanes})
+                if ((st = trial(m, l, false))) break;
+        }
+        if (st) {
+            A->lanes = upload_lanes;
+            A->use_panel = A->pw && !A->plain;
+            return st;
         }
         g->methods[i] = best_m;
         A->lanes = best_l;
+        A->use_panel = best_panel;
+        // Keep only the encoding that won.
+        if (A->pw && A->plain) drop_encoding(A, !best_panel);
     }
     return CLSPMV_OK;
 }

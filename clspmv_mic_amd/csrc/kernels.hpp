@@ -28,6 +28,24 @@ hipError_t coo_segreduce(hipStream_t s, int nnz, const int* row, const int* col,
                          const float* val, const float* x, float* y,
                          int* carry_row, float* carry_val);
 
+// ---- x-panel encoding (COO and CSR) -------------------------------------------
+// Entries grouped by column panel (kPanelMax columns at most, so a panel's x
+// fits in LDS), row-sorted inside each panel, padded per panel to whole
+// sub-chunks of kPanelSub entries.  Each entry is a 16-bit word (bit 15:
+// first entry of a segment, bits 0-14: column inside the panel) and a float.
+// A segment is a run of one row's entries inside one sub-chunk; pass 1
+// writes one partial sum per segment, pass 2 adds each row's segments to y
+// in panel order.  `wg` holds one int4 per work-group: (panel, first
+// sub-chunk, end sub-chunk, end of the panel's real entries).
+constexpr int kPanelSub = 256;        // entries per wave-sized sub-chunk
+constexpr int kPanelMax = 32760;      // largest panel width (floats)
+constexpr int kPanelThreads = 1024;   // pass-1 work-group size
+hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
+                      const unsigned short* pcol, const float* pval,
+                      const int* sc_seg, const float* x, bool x_aligned,
+                      float* seg_val, int nrows, const int* rrow,
+                      const int* rs_ptr, const int* rs_idx, float* y);
+
 // ---- ELL / SELL --------------------------------------------------------------
 hipError_t ell(hipStream_t s, int rows, int hal, int ellnum, int rows_per_lane,
                const int* col, const float* val, const float* x, float* y);

@@ -105,6 +105,15 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
     if (I.rows == 0 || I.cols == 0 || I.nnz == 0) return CLSPMV_OK;
     hipError_t e = hipSuccess;
     const bool xa = aligned16(x);
+    if ((I.format == CLSPMV_FMT_CSR || I.format == CLSPMV_FMT_COO) && A->use_panel) {
+        // The panel encoding serves every method of the format.
+        e = launch::panel_spmv(s, A->pnwg, A->pwg, A->pw, I.cols, A->pcol, A->pval, A->psc_seg, x, xa,
+                               A->pseg, A->prows, A->prrow, A->prs_ptr, A->prs_idx, y);
+        if (e != hipSuccess)
+            return fail(CLSPMV_ERR_HIP, "panel launch failed (format %d): %s", I.format,
+                        hipGetErrorString(e));
+        return CLSPMV_OK;
+    }
     switch (I.format) {
         case CLSPMV_FMT_CSR:
             e = method == 0 ? launch::csr_vector(s, I.rows, A->lanes, A->rp, A->col, A->val, x, y)
@@ -306,7 +315,44 @@ int clspmv_method_count(int format) {
         }                             \
     } while (0)
 
+// Builds the panel encoding asked for by `panel_width` (a width, or
+// CLSPMV_PANEL_AUTO / CLSPMV_PANEL_BOTH) from row-sorted entries; with AUTO
+// it replaces the plain arrays, with BOTH it is added beside them.
+static int add_panel(clspmv_dmat* d, int panel_width, const int* row, const int* col,
+                     const float* val) {
+    const clspmv_dmat_info& I = d->info;
+    int pw = panel_width;
+    if (pw == CLSPMV_PANEL_AUTO || pw == CLSPMV_PANEL_BOTH) pw = auto_panel_width(I.cols, I.nnz);
+    if (pw == 0) return CLSPMV_OK;
+    int st = build_panel(d, I.rows, I.cols, I.nnz, row, col, val, pw);
+    if (st) return st;
+    if (panel_width != CLSPMV_PANEL_BOTH) drop_encoding(d, false);
+    return CLSPMV_OK;
+}
+
+static int check_panel_arg(int panel_width) {
+    if (panel_width == 0 || panel_width == CLSPMV_PANEL_AUTO || panel_width == CLSPMV_PANEL_BOTH)
+        return CLSPMV_OK;
+    if (panel_width < 8 || panel_width % 8 || panel_width > launch::kPanelMax)
+        return fail(CLSPMV_ERR_INVALID, "panel width %d: need 0, -1, -2 or a multiple of 8 in [8, %d]",
+                    panel_width, launch::kPanelMax);
+    return CLSPMV_OK;
+}
+
 int clspmv_upload_csr(clspmv_ctx* ctx, const clspmv_csr_matrix* m, clspmv_dmat** out) {
+    return clspmv_upload_csr_ex(ctx, m, 0, out);
+}
+
+int clspmv_upload_coo(clspmv_ctx* ctx, const clspmv_coo_matrix* m, clspmv_dmat** out) {
+    return clspmv_upload_coo_ex(ctx, m, CLSPMV_PANEL_AUTO, out);
+}
+
+int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m, int panel_width,
+                         clspmv_dmat** out) {
+    {
+        int st0 = check_panel_arg(panel_width);
+        if (st0) return st0;
+    }
     UPLOAD_PROLOGUE(m, CLSPMV_FMT_CSR)
     const int rows = m->matinfo.height, nnz = m->matinfo.nnz;
     if (rows < 0 || nnz < 0 || (rows > 0 && !m->csr_row_ptr))
@@ -330,11 +376,22 @@ int clspmv_upload_csr(clspmv_ctx* ctx, const clspmv_csr_matrix* m, clspmv_dmat**
     std::vector<int> blocks = adaptive_blocks(m->csr_row_ptr, rows);
     d->nblocks = (int)blocks.size() - 1;
     UPLOAD_CHECK(upload_array(d, blocks.data(), blocks.size(), &d->blocks));
+    if (panel_width != 0 && nnz > 0) {
+        std::vector<int> row_of(nnz);
+        for (int r = 0; r < rows; ++r)
+            std::fill(row_of.begin() + m->csr_row_ptr[r], row_of.begin() + m->csr_row_ptr[r + 1], r);
+        UPLOAD_CHECK(add_panel(d, panel_width, row_of.data(), m->csr_col_id, m->csr_data));
+    }
     *out = d;
     return CLSPMV_OK;
 }
 
-int clspmv_upload_coo(clspmv_ctx* ctx, const clspmv_coo_matrix* m, clspmv_dmat** out) {
+int clspmv_upload_coo_ex(clspmv_ctx* ctx, const clspmv_coo_matrix* m, int panel_width,
+                         clspmv_dmat** out) {
+    {
+        int st0 = check_panel_arg(panel_width);
+        if (st0) return st0;
+    }
     UPLOAD_PROLOGUE(m, CLSPMV_FMT_COO)
     UPLOAD_CHECK(clspmv_validate_coo(m));
     const int nnz = m->matinfo.nnz;
@@ -358,6 +415,8 @@ int clspmv_upload_coo(clspmv_ctx* ctx, const clspmv_coo_matrix* m, clspmv_dmat**
     const int chunks = (nnz + launch::kCooChunk - 1) / launch::kCooChunk;
     UPLOAD_CHECK(upload_array<int>(d, nullptr, 2 * (size_t)chunks, &d->carry_row));
     UPLOAD_CHECK(upload_array<float>(d, nullptr, 2 * (size_t)chunks, &d->carry_val));
+    if (panel_width != 0 && nnz > 0)
+        UPLOAD_CHECK(add_panel(d, panel_width, sorted.coo_row_id, sorted.coo_col_id, sorted.coo_data));
     *out = d;
     return CLSPMV_OK;
 }
@@ -538,7 +597,7 @@ int clspmv_dmat_free(clspmv_dmat* m) {
     int cur = -1;
     (void)hipGetDevice(&cur);
     if (cur != m->device) (void)hipSetDevice(m->device);
-    for (void* p : m->bufs) (void)hipFree(p);
+    for (auto& b : m->bufs) (void)hipFree(b.first);
     if (cur != m->device && cur >= 0) (void)hipSetDevice(cur);
     delete m;
     return CLSPMV_OK;
@@ -547,6 +606,17 @@ int clspmv_dmat_free(clspmv_dmat* m) {
 int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out) {
     if (!m || !out) return fail(CLSPMV_ERR_INVALID, "null argument");
     *out = m->info;
+    out->panel_width = m->pw;
+    out->panel_active = m->use_panel ? 1 : 0;
+    out->plain = m->plain ? 1 : 0;
+    return CLSPMV_OK;
+}
+
+int clspmv_dmat_set_panel(clspmv_dmat* m, int use_panel) {
+    if (!m) return fail(CLSPMV_ERR_INVALID, "null argument");
+    if (use_panel && !m->pw) return fail(CLSPMV_ERR_UNSUPPORTED, "matrix has no panel encoding");
+    if (!use_panel && !m->plain) return fail(CLSPMV_ERR_UNSUPPORTED, "matrix has only the panel encoding");
+    m->use_panel = use_panel != 0;
     return CLSPMV_OK;
 }
 

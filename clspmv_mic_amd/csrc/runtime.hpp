@@ -18,7 +18,7 @@ struct clspmv_ctx {
 struct clspmv_dmat {
     clspmv_dmat_info info{};
     int device = 0;
-    std::vector<void*> bufs;  // device allocations owned by the matrix
+    std::vector<std::pair<void*, size_t>> bufs;  // device allocations owned by the matrix
 
     // CSR: rp/col/val; adaptive row blocks; vector lanes per row.
     int* rp = nullptr;
@@ -42,6 +42,19 @@ struct clspmv_dmat {
     int len = 0, ndiag = 0, nband = 0, min_off = 0, max_off = 0, max_band = 0;
     // Block formats: block rows, aligned sizes, group lanes.
     int brows = 0, aligned = 0, f4 = 0;
+    // x-panel encoding of COO / CSR (pw == 0: absent).  plain == false when
+    // the (row, col, val) / CSR arrays were not uploaded.
+    int pw = 0, pnwg = 0, prows = 0;
+    int* pwg = nullptr;  // int4 per work-group
+    unsigned short* pcol = nullptr;
+    float* pval = nullptr;
+    int* psc_seg = nullptr;
+    float* pseg = nullptr;
+    int* prrow = nullptr;
+    int* prs_ptr = nullptr;
+    int* prs_idx = nullptr;
+    bool use_panel = false;
+    bool plain = true;
 };
 
 namespace clspmv {
@@ -63,7 +76,7 @@ int upload_array(clspmv_dmat* m, const T* host, size_t n, T** out) {
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess)
         return fail(CLSPMV_ERR_ALLOC, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    m->bufs.push_back(p);
+    m->bufs.push_back({p, bytes});
     m->info.device_bytes += bytes;
     if (n && host) {
         e = hipMemcpy(p, host, sizeof(T) * n, hipMemcpyHostToDevice);
@@ -73,6 +86,32 @@ int upload_array(clspmv_dmat* m, const T* host, size_t n, T** out) {
     *out = static_cast<T*>(p);
     return CLSPMV_OK;
 }
+
+// Frees one allocation of `m` (no-op for null); the pointer is cleared.
+template <class T>
+void release_array(clspmv_dmat* m, T** p) {
+    if (!*p) return;
+    for (size_t i = 0; i < m->bufs.size(); ++i)
+        if (m->bufs[i].first == (void*)*p) {
+            (void)hipFree(m->bufs[i].first);
+            m->info.device_bytes -= m->bufs[i].second;
+            m->bufs.erase(m->bufs.begin() + i);
+            break;
+        }
+    *p = nullptr;
+}
+
+// x-panel encoding (k_panel.hip) of row-sorted entries, built on the host
+// and uploaded into `d`.  pw: panel width (multiple of 8, at most
+// launch::kPanelMax).
+int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* row,
+                const int* col, const float* val, int pw);
+// Panel width the encoding rule picks for a matrix, or 0 when panels do not
+// pay (too few nonzeros per panel to amortize staging x in LDS).
+int auto_panel_width(int cols, long long nnz);
+// Drops one of two uploaded encodings: the panel one (panel == true) or the
+// plain arrays.
+void drop_encoding(clspmv_dmat* d, bool panel);
 
 // Checks that the context's device is current; sets it otherwise.
 int use_ctx_device(clspmv_ctx* ctx);

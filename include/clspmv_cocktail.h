@@ -103,7 +103,9 @@ typedef struct clspmv_cocktail_kernels clspmv_cocktail_kernels;
 /* init_mat_kernels (src/spmv_cocktail.cpp:1590): uploads every part of
  * cpumat to device `device_ordinal`, x from vec (mat_width floats) and y
  * from res (mat_height floats), and prepares the launch sequence, captured
- * once into a HIP graph.  The cocktail is not retained. */
+ * once into a HIP graph.  CSR and COO parts are uploaded with
+ * CLSPMV_PANEL_BOTH (the panel encoding active where it pays) until
+ * clspmv_cocktail_tune keeps one.  The cocktail is not retained. */
 int clspmv_init_mat_kernels(const clspmv_cocktail* cpumat, const float* vec,
                             const float* res, clspmv_cocktail_gpu** gpumat,
                             clspmv_cocktail_kernels** kernels, int use_tex);
@@ -156,7 +158,9 @@ int clspmv_cocktail_profile(clspmv_cocktail_kernels* kernels, int ntimes,
  * runtime variant probe (init_mat_kernels with use_tex,
  * src/spmv_cocktail.cpp:1590): every method a part's shape allows (and,
  * for CSR method 0 and BCSR, the neighbouring lane counts) is timed with
- * HIP events over ntimes launches and the fastest kept; then the captured
+ * HIP events over ntimes launches and the fastest kept (CSR and COO parts
+ * are uploaded in both their own and the x-panel encoding, and the losing
+ * encoding is freed); then the captured
  * graph is re-built and compared with plain stream launches of the same
  * pass, and the faster launch mode is kept.  y is preserved.  Tuning picks
  * by timing, so two processes may settle on different methods whose

@@ -82,6 +82,9 @@ typedef struct {
     size_t device_bytes;    /* device memory held by the matrix         */
     int    bheight, bwidth; /* block shape (block formats), else 1,1    */
     int    slice_height;    /* SELL / SBELL slice height, else 0        */
+    int    panel_width;     /* COO / CSR x-panel encoding width, or 0   */
+    int    panel_active;    /* 1 when launches use the panel encoding   */
+    int    plain;           /* 1 when the format's own arrays are held  */
 } clspmv_dmat_info;
 
 /* Upload a host matrix (the reference's cpy_<fmt>_mat,
@@ -96,6 +99,33 @@ int clspmv_upload_b4csr(clspmv_ctx* ctx, const clspmv_b4csr_matrix* m, clspmv_dm
 int clspmv_upload_b4ell(clspmv_ctx* ctx, const clspmv_b4ell_matrix* m, clspmv_dmat** out);
 int clspmv_upload_sbell(clspmv_ctx* ctx, const clspmv_sbell_matrix* m, clspmv_dmat** out);
 int clspmv_dmat_free(clspmv_dmat* m);
+
+/* COO and CSR uploads with a choice of device encoding.  Besides the
+ * format's own arrays the engine has an x-panel encoding for matrices whose
+ * column indices have no locality: columns are cut into panels of
+ * panel_width floats (a multiple of 8, at most 32760) so that a work-group
+ * holds its panel of x in LDS, and entries are stored as 16-bit in-panel
+ * columns with segment marks (6 bytes per nonzero).  Results are the same
+ * to rounding; the summation order is fixed by the encoding.
+ *   panel_width 0                  the format's own arrays only
+ *   panel_width > 0                the panel encoding only, that width
+ *   CLSPMV_PANEL_AUTO (-1)         panels when the matrix has enough
+ *                                  nonzeros per panel, else the own arrays
+ *   CLSPMV_PANEL_BOTH (-2)         both encodings where panels pay (panel
+ *                                  active); clspmv_dmat_set_panel switches
+ * clspmv_upload_coo is clspmv_upload_coo_ex(..., CLSPMV_PANEL_AUTO, ...),
+ * clspmv_upload_csr is clspmv_upload_csr_ex(..., 0, ...).  With the panel
+ * encoding active, every method number of the format runs the panel
+ * kernels. */
+#define CLSPMV_PANEL_AUTO (-1)
+#define CLSPMV_PANEL_BOTH (-2)
+int clspmv_upload_coo_ex(clspmv_ctx* ctx, const clspmv_coo_matrix* m, int panel_width,
+                         clspmv_dmat** out);
+int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m, int panel_width,
+                         clspmv_dmat** out);
+/* Selects the panel encoding (1) or the format's own arrays (0) of a matrix
+ * that holds both. */
+int clspmv_dmat_set_panel(clspmv_dmat* m, int use_panel);
 int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out);
 
 /* y += A*x on the context's stream (asynchronous).  x holds cols floats,

@@ -45,7 +45,7 @@ def _is_dyadic(a: np.ndarray) -> bool:
     return bool(np.all(a * 64 == np.round(a * 64)))
 
 
-def _run(ctx, oracle, A: cs.Coo, owned, method):
+def _run(ctx, oracle, A: cs.Coo, owned, method, panel=None, x_offset=0):
     dyadic = _is_dyadic(A.val)
     if dyadic:
         x = matrices.dyadic_vector(A.width, seed=11)
@@ -53,8 +53,15 @@ def _run(ctx, oracle, A: cs.Coo, owned, method):
     else:
         x = matrices.uniform_vector(A.width, seed=11)
         y0 = matrices.uniform_vector(A.height, seed=12)
-    dA = cs.DeviceMatrix(ctx, owned)
-    dx = cs.DeviceVector(ctx, A.width, x)
+    dA = cs.DeviceMatrix(ctx, owned, panel=panel)
+    if panel is not None and panel > 0:
+        assert dA.info.panel_width == panel and dA.info.panel_active == 1 and dA.info.plain == 0
+    if x_offset:
+        # x starting x_offset floats into its buffer: unaligned for 128-bit loads
+        dx = cs.DeviceVector(ctx, A.width + x_offset, np.concatenate([np.zeros(x_offset, np.float32), x]))
+        dx = dx.view(x_offset)
+    else:
+        dx = cs.DeviceVector(ctx, A.width, x)
     dy = cs.DeviceVector(ctx, A.height, y0)
     dA.spmv(dx, dy, method)
     ctx.sync()
@@ -199,3 +206,52 @@ def test_repeat_is_deterministic(ctx):
             outs.append(dy.download())
         np.testing.assert_array_equal(outs[0], outs[1])
         np.testing.assert_array_equal(outs[0], outs[2])
+
+
+PANEL_CASES = ["rand_small", "rand_dyadic", "rand_empty_rows", "rand_long_rows", "rand_long_dyadic",
+               "lap2d_256", "tall", "wide", "banded_dyadic"]
+
+
+@pytest.mark.parametrize("name", PANEL_CASES)
+@pytest.mark.parametrize("pw", [8, 64, 1024, 32760])
+@pytest.mark.parametrize("fmt", ["coo", "csr"])
+def test_panel_encoding(ctx, oracle, name, pw, fmt):
+    """x-panel encoding (16-bit in-panel columns, LDS-staged x) at widths from
+    one to many panels, COO and CSR, both methods of CSR."""
+    A = CASES[name]
+    owned = A.sorted() if fmt == "coo" else cs.coo2csr(A.copy())
+    for method in ([0] if fmt == "coo" else [0, 1]):
+        _run(ctx, oracle, A, owned, method, panel=pw)
+
+
+@pytest.mark.parametrize("fmt", ["coo", "csr"])
+def test_panel_unaligned_x(ctx, oracle, fmt):
+    A = CASES["rand_long_rows"]
+    owned = A.sorted() if fmt == "coo" else cs.coo2csr(A.copy())
+    _run(ctx, oracle, A, owned, 0, panel=64, x_offset=1)
+
+
+def test_panel_both_encodings_switch(ctx, oracle):
+    """PANEL_BOTH keeps both encodings; each gives the oracle's result."""
+    A = matrices.random_sparse(3000, 70000, 0.002, seed=21, long_rows=4, long_len=3000)
+    for fmt in ("coo", "csr"):
+        owned = A.sorted() if fmt == "coo" else cs.coo2csr(A.copy())
+        x = matrices.uniform_vector(A.width, seed=11)
+        y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height)
+        dA = cs.DeviceMatrix(ctx, owned, panel=cs.PANEL_BOTH)
+        assert dA.info.panel_width > 0 and dA.info.plain == 1
+        for use in (True, False):
+            dA.set_panel(use)
+            dx = cs.DeviceVector(ctx, A.width, x)
+            dy = cs.DeviceVector(ctx, A.height, np.zeros(A.height, np.float32))
+            dA.spmv(dx, dy, 0)
+            ctx.sync()
+            err, worst = oracle.scaled_error(dy.download(), y64, bound)
+            assert err <= TOL, f"{fmt} panel={use}: {err:.3e} at {worst}"
+
+
+def test_panel_bad_width_rejected(ctx):
+    A = CASES["rand_small"].sorted()
+    for bad in (4, 12, 32768, -3):
+        with pytest.raises(cs.ClspmvError):
+            cs.DeviceMatrix(ctx, A, panel=bad)
