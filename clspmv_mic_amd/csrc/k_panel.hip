@@ -10,11 +10,13 @@
 // is 6 bytes per nonzero instead of 12 (COO) or 8 (CSR).
 //
 // Pass 1: each wave takes sub-chunks of 256 entries (4 per lane), forms the
-// products, and reduces every segment (one row's run inside the sub-chunk)
-// with a segmented scan over the wave's lanes; each segment's sum goes to
-// seg_val.  Pass 2: one lane per row with segments adds them in panel order
-// and then adds the total to y.  The summation order is fixed by the
-// encoding alone; no atomics.
+// products and reduces every segment (one row's run inside the sub-chunk)
+// with a segmented scan over the wave's lanes (DPP row shifts and
+// broadcasts); each segment's sum goes to seg_val.  Pass 2: one work-group
+// per row block loads the block's segments (one contiguous range, ordered
+// by panel then row), adds them to per-row LDS accumulators panel by panel,
+// and adds the totals to y.  The summation order is fixed by the encoding;
+// no atomics.
 
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -28,14 +30,46 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kU = 2;  // sub-chunks in flight per wave
 
+// DPP controls (GFX9 encoding): shifts within 16-lane rows, broadcasts of
+// lane 15 / 31 to the following rows, and a whole-wave shift by one.
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
+
+// The lane's value moved by DPP; lanes with no source (or in rows outside
+// ROW_MASK) receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_float(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, true));
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_int(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, true);
+}
+
+// One step of a segmented inclusive scan: (s, g) absorbs the partner's
+// (s, g) unless this lane already starts a segment.  A partner of 0 / 0
+// changes nothing, so lanes without a source are left as they are.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void seg_step(float& s, int& g) {
+    const float ts = dpp_float<CTRL, ROW_MASK>(s);
+    const int tg = dpp_int<CTRL, ROW_MASK>(g);
+    if (!g) s += ts;
+    g |= tg;
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ int lanes_below(unsigned long long mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
 struct SubChunk {
     u32x2 c;   // four 16-bit entry words
     f32x4 v;   // four values
 };
 
-__device__ __forceinline__ void seg_scan_write(const SubChunk& q, int i0, int real_end,
-                                               const float* __restrict__ xs, int lane,
-                                               float* __restrict__ out) {
+__device__ __forceinline__ void seg_scan_write(const SubChunk& q, const float* __restrict__ xs,
+                                               int lane, float* __restrict__ out) {
     const unsigned w[4] = {q.c.x & 0xFFFFu, q.c.x >> 16, q.c.y & 0xFFFFu, q.c.y >> 16};
     const float v[4] = {q.v.x, q.v.y, q.v.z, q.v.w};
     int f[4];
@@ -43,14 +77,16 @@ __device__ __forceinline__ void seg_scan_write(const SubChunk& q, int i0, int re
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         f[k] = (int)(w[k] >> 15);
-        p[k] = (i0 + k < real_end) ? v[k] * xs[w[k] & 0x7FFFu] : 0.0f;
+        const unsigned c = w[k] & 0x7FFFu;
+        const bool real = c != launch::kPanelPad;
+        const float xv = xs[real ? c : 0u];
+        p[k] = real ? v[k] * xv : 0.0f;
     }
     if (lane == 0) f[0] = 1;  // every sub-chunk opens a segment
     // Segment numbering: flags of earlier lanes, then this lane's own.
-    const unsigned long long lt = (1ull << lane) - 1ull;
     int before = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) before += __popcll(__ballot(f[k]) & lt);
+    for (int k = 0; k < 4; ++k) before += lanes_below(__ballot(f[k]));
     const int nf = f[0] + f[1] + f[2] + f[3];
 
     // Runs inside the lane: head (before the first flag), whole segments,
@@ -75,15 +111,14 @@ __device__ __forceinline__ void seg_scan_write(const SubChunk& q, int i0, int re
     // its tail when it holds a flag, else its whole sum.
     float s = nf ? run : head;
     int g = nf ? 1 : 0;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const float us = __shfl_up(s, d, kWave);
-        const int ug = __shfl_up(g, d, kWave);
-        if (lane >= d && !g) s += us;
-        if (lane >= d) g |= ug;
-    }
+    seg_step<kRowShr1, 0xF>(s, g);
+    seg_step<kRowShr2, 0xF>(s, g);
+    seg_step<kRowShr4, 0xF>(s, g);
+    seg_step<kRowShr8, 0xF>(s, g);
+    seg_step<kRowBcast15, 0xA>(s, g);
+    seg_step<kRowBcast31, 0xC>(s, g);
     // The segment open before this lane ends at its first flag.
-    const float prev = __shfl_up(s, 1, kWave);
+    const float prev = dpp_float<kWaveShr1, 0xF>(s);
     if (nf && lane > 0) out[before - 1] = prev + head;
     // The last lane closes the sub-chunk's last segment.
     if (lane == kWave - 1) out[before + nf - 1] = s;
@@ -93,74 +128,124 @@ template <int NT>
 __global__ __launch_bounds__(NT) void panel_pass1_kernel(
     const int4* __restrict__ wg, int pw, int cols,
     const unsigned short* __restrict__ pcol, const float* __restrict__ pval,
-    const int* __restrict__ sc_seg, const float* __restrict__ x, bool xa,
-    float* __restrict__ seg_val) {
+    const int* __restrict__ sc_seg, const int* __restrict__ sc_start,
+    const float* __restrict__ x, bool xa, float* __restrict__ seg_val) {
     extern __shared__ float xs[];
     const int4 t = wg[blockIdx.x];
-    const int panel = t.x, sc0 = t.y, sc1 = t.z, real_end = t.w;
+    const int panel = t.x, sc0 = t.y, sc1 = t.z;
     const int xb = panel * pw;
     const int w = min(pw, cols - xb);
     const int tid = threadIdx.x;
-    if (xa) {
-        for (int i = tid * 4; i < w; i += NT * 4) {
-            if (i + 3 < w) {
-                const float4 v = *reinterpret_cast<const float4*>(x + xb + i);
-                xs[i] = v.x;
-                xs[i + 1] = v.y;
-                xs[i + 2] = v.z;
-                xs[i + 3] = v.w;
-            } else {
-                for (int k = i; k < w; ++k) xs[k] = x[xb + k];
-            }
+    // Stage the panel: every load is issued before the first LDS store, so
+    // the work-group waits for one memory latency, not one per step.
+    constexpr int kStage = (launch::kPanelMax + 4 * NT - 1) / (4 * NT);
+    const int w4 = w & ~3;  // elements staged with 128-bit loads
+    if (xa && w4 >= 4) {
+        // Unconditional loads (indices clamped into the panel) keep the
+        // loop free of branches, so no wait separates them.
+        f32x4 v[kStage];
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int i = min((tid + k * NT) * 4, w4 - 4);
+            v[k] = *reinterpret_cast<const f32x4*>(x + xb + i);
         }
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int i = (tid + k * NT) * 4;
+            if (i < w4) *reinterpret_cast<f32x4*>(xs + i) = v[k];
+        }
+        for (int i = w4 + tid; i < w; i += NT) xs[i] = x[xb + i];
     } else {
-        for (int i = tid; i < w; i += NT) xs[i] = x[xb + i];
+        float v[4 * kStage];
+#pragma unroll
+        for (int k = 0; k < 4 * kStage; ++k) {
+            const int i = min(tid + k * NT, w - 1);
+            v[k] = x[xb + i];
+        }
+#pragma unroll
+        for (int k = 0; k < 4 * kStage; ++k) {
+            const int i = tid + k * NT;
+            if (i < w) xs[i] = v[k];
+        }
     }
     __syncthreads();
 
-    const int wave = tid / kWave, lane = tid & (kWave - 1);
+    // Sub-chunk indices are wave-uniform: keep them in scalar registers.
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int lane = tid & (kWave - 1);
     constexpr int nw = NT / kWave;
-    for (int sc = sc0 + wave; sc < sc1; sc += nw * kU) {
-        SubChunk q[kU];
+    // Software pipeline: the loads of the next kU sub-chunks are in flight
+    // while the current ones are reduced.
+    SubChunk cur[kU], nxt[kU];
+    auto load = [&](SubChunk* q, int sc) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int c = sc + u * nw;
+            // Lanes past a short sub-chunk's end hold padding.
+            q[u].c = u32x2{0x7FFF7FFFu, 0x7FFF7FFFu};
+            q[u].v = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             if (c < sc1) {
-                const int i0 = c * launch::kPanelSub + lane * 4;
-                q[u].c = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(pcol + i0));
-                q[u].v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(pval + i0));
+                const int i0 = sc_start[c] + lane * 4;
+                if (i0 < sc_start[c + 1]) {
+                    q[u].c = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(pcol + i0));
+                    q[u].v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(pval + i0));
+                }
             }
         }
+    };
+    int sc = sc0 + wave;
+    load(cur, sc);
+    for (; sc < sc1; sc += nw * kU) {
+        load(nxt, sc + nw * kU);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int c = sc + u * nw;
-            if (c < sc1)
-                seg_scan_write(q[u], c * launch::kPanelSub + lane * 4, real_end, xs, lane,
-                               seg_val + sc_seg[c]);
+            if (c < sc1) seg_scan_write(cur[u], xs, lane, seg_val + sc_seg[c]);
         }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
     }
 }
 
-__global__ __launch_bounds__(kBlock) void panel_pass2_kernel(
-    int nrows, const int* __restrict__ rrow, const int* __restrict__ rs_ptr,
-    const int* __restrict__ rs_idx, const float* __restrict__ seg_val,
+__global__ __launch_bounds__(launch::kBlockThreads) void panel_pass2_kernel(
+    int npanels, const int* __restrict__ blk_row, const int* __restrict__ blk_tab,
+    const int* __restrict__ seg_row, const float* __restrict__ seg_val,
     float* __restrict__ y) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= nrows) return;
-    const int s = rs_ptr[i], e = rs_ptr[i + 1];
-    float acc = 0.0f;
-    int j = s;
-    // Rows of long power-law tails hold dozens of segments: issue eight
-    // independent loads at a time, then add them in order.
-    for (; j + 8 <= e; j += 8) {
-        float v[8];
+    constexpr int NT = launch::kBlockThreads, K = launch::kBlockItems;
+    __shared__ float acc[launch::kBlockRowsMax];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int row0 = blk_row[b], nrows = blk_row[b + 1] - row0;
+    const int* tab = blk_tab + (size_t)b * (npanels + 1);
+    const int j0 = tab[0], j1 = tab[npanels];
+    // All of the block's segments are loaded before any is used.
+    int r[K];
+    float v[K];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = seg_val[rs_idx[j + k]];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) acc += v[k];
+    for (int k = 0; k < K; ++k) {
+        const int j = j0 + tid + k * NT;
+        r[k] = j < j1 ? seg_row[j] : -1;
+        v[k] = j < j1 ? seg_val[j] : 0.0f;
     }
-    for (; j < e; ++j) acc += seg_val[rs_idx[j]];
-    y[rrow[i]] += acc;
+    // -0.0 is the identity of +: rows without segments keep y bit for bit.
+    for (int i = tid; i < nrows; i += NT) acc[i] = -0.0f;
+    __syncthreads();
+    for (int p = 0; p < npanels; ++p) {
+        const int lo = tab[p], hi = tab[p + 1];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = j0 + tid + k * NT;
+            // A continuation segment (bit 31) is added by its run's head;
+            // bit 30 tells the head that one follows.
+            if (j >= lo && j < hi && r[k] >= 0) {
+                float sum = v[k];
+                if (r[k] & 0x40000000)
+                    for (int q = j + 1; q < hi && seg_row[q] < 0; ++q) sum += seg_val[q];
+                acc[(r[k] & 0x3FFFFFFF) - row0] += sum;
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < nrows; i += NT) y[row0 + i] += acc[i];
 }
 
 }  // namespace
@@ -169,10 +254,10 @@ namespace launch {
 
 hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
                       const unsigned short* pcol, const float* pval,
-                      const int* sc_seg, const float* x, bool x_aligned,
-                      float* seg_val, int nrows, const int* rrow,
-                      const int* rs_ptr, const int* rs_idx, float* y) {
-    if (nwg <= 0 || nrows <= 0) return hipSuccess;
+                      const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
+                      float* seg_val, int npanels, int nblocks, const int* blk_row,
+                      const int* blk_tab, const int* seg_row, float* y) {
+    if (nwg <= 0 || nblocks <= 0) return hipSuccess;
     if (pw <= 0 || pw > kPanelMax) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * (size_t)pw;
     static bool attr_set = false;
@@ -184,12 +269,12 @@ hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
         attr_set = true;
     }
     hipLaunchKernelGGL(panel_pass1_kernel<kPanelThreads>, dim3(nwg), dim3(kPanelThreads), lds, s,
-                       reinterpret_cast<const int4*>(wg), pw, cols, pcol, pval, sc_seg, x,
-                       x_aligned, seg_val);
+                       reinterpret_cast<const int4*>(wg), pw, cols, pcol, pval, sc_seg, sc_start,
+                       x, x_aligned, seg_val);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(panel_pass2_kernel, dim3((nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       nrows, rrow, rs_ptr, rs_idx, seg_val, y);
+    hipLaunchKernelGGL(panel_pass2_kernel, dim3(nblocks), dim3(kBlockThreads), 0, s, npanels,
+                       blk_row, blk_tab, seg_row, seg_val, y);
     return hipGetLastError();
 }
 

@@ -29,22 +29,34 @@ hipError_t coo_segreduce(hipStream_t s, int nnz, const int* row, const int* col,
                          int* carry_row, float* carry_val);
 
 // ---- x-panel encoding (COO and CSR) -------------------------------------------
-// Entries grouped by column panel (kPanelMax columns at most, so a panel's x
-// fits in LDS), row-sorted inside each panel, padded per panel to whole
-// sub-chunks of kPanelSub entries.  Each entry is a 16-bit word (bit 15:
-// first entry of a segment, bits 0-14: column inside the panel) and a float.
-// A segment is a run of one row's entries inside one sub-chunk; pass 1
-// writes one partial sum per segment, pass 2 adds each row's segments to y
-// in panel order.  `wg` holds one int4 per work-group: (panel, first
-// sub-chunk, end sub-chunk, end of the panel's real entries).
+// Columns are cut into panels of at most kPanelMax floats (a panel's x fits
+// in LDS) and rows into blocks.  Entries are grouped by (panel, row block),
+// row-sorted inside a group; a group is padded to a multiple of 4 entries
+// and cut into sub-chunks of kPanelSub entries (the last one shorter;
+// sc_start holds each sub-chunk's first entry).  An entry is a 16-bit word
+// (bit 15: first entry of a segment; bits 0-14: column inside the panel,
+// kPanelPad for padding) and a float.  A segment is one row's run inside
+// one sub-chunk; seg_row holds its row, bit 31 when it continues the
+// previous segment's row and bit 30 when the next segment continues it.
+// Pass 1 (one work-group per run of sub-chunks of one panel, `wg` holds
+// (panel, first sub-chunk, end sub-chunk, 0) per group) writes each
+// segment's sum to seg_val[sc_seg[sub-chunk] + k].  Segments are numbered
+// in (row block, panel, row) order, so pass 2 (one work-group per row block)
+// reads its block's segments as one contiguous range, adds them per row in
+// panel order in LDS, and adds each row's total to y.
 constexpr int kPanelSub = 256;        // entries per wave-sized sub-chunk
 constexpr int kPanelMax = 32760;      // largest panel width (floats)
+constexpr unsigned kPanelPad = 0x7FFF;  // column code of padding entries
 constexpr int kPanelThreads = 1024;   // pass-1 work-group size
+constexpr int kBlockRowsMax = 4096;   // rows of a pass-2 row block
+constexpr int kBlockThreads = 1024;   // pass-2 work-group size
+constexpr int kBlockItems = 8;        // segments per pass-2 thread
+constexpr int kBlockSegsMax = kBlockThreads * kBlockItems;
 hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
                       const unsigned short* pcol, const float* pval,
-                      const int* sc_seg, const float* x, bool x_aligned,
-                      float* seg_val, int nrows, const int* rrow,
-                      const int* rs_ptr, const int* rs_idx, float* y);
+                      const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
+                      float* seg_val, int npanels, int nblocks, const int* blk_row,
+                      const int* blk_tab, const int* seg_row, float* y);
 
 // ---- ELL / SELL --------------------------------------------------------------
 hipError_t ell(hipStream_t s, int rows, int hal, int ellnum, int rows_per_lane,

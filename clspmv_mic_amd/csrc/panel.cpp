@@ -1,6 +1,6 @@
 // Host side of the x-panel encoding (k_panel.hip): regroups row-sorted
 // entries by column panel, marks segment starts, numbers the segments and
-// builds the per-row segment lists and the work-group table.
+// builds the row-block tables and the work-group table.
 
 #include <algorithm>
 #include <climits>
@@ -28,88 +28,176 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
                     launch::kPanelMax);
     if (cols <= 0 || rows <= 0 || nnz <= 0) return CLSPMV_OK;
     const int P = (cols + pw - 1) / pw;
-    std::vector<long long> cnt(P, 0);
-    for (long long i = 0; i < nnz; ++i) cnt[col[i] / pw]++;
-    std::vector<long long> start(P + 1, 0);
-    for (int p = 0; p < P; ++p) start[p + 1] = start[p] + (cnt[p] + kSub - 1) / kSub * kSub;
-    const long long np = start[P];
+
+    // Row blocks for pass 2: at most kBlockRowsMax rows and, bounding each
+    // row's segments by its per-panel runs (a run of L > 1 entries becomes
+    // at most L/256 + 2 segments), at most kBlockSegsMax segments.
+    if (rows >= (1 << 30)) return fail(CLSPMV_ERR_UNSUPPORTED, "panel encoding needs rows < 2^30");
+    std::vector<int> blk_row(1, 0);
+    {
+        long long bound_sum = 0;
+        int r0 = 0;
+        long long i = 0;
+        for (int r = 0; r < rows; ++r) {
+            long long bound = 0;
+            while (i < nnz && row[i] == r) {
+                const int p = col[i] / pw;
+                long long run = 0;
+                while (i < nnz && row[i] == r && col[i] / pw == p) {
+                    ++run;
+                    ++i;
+                }
+                bound += run == 1 ? 1 : run / kSub + 2;
+            }
+            if (bound > launch::kBlockSegsMax)
+                return fail(CLSPMV_ERR_UNSUPPORTED, "row %d is too long for the panel encoding", r);
+            if (r - r0 == launch::kBlockRowsMax || bound_sum + bound > launch::kBlockSegsMax) {
+                blk_row.push_back(r);
+                r0 = r;
+                bound_sum = 0;
+            }
+            bound_sum += bound;
+        }
+        if (i != nnz) return fail(CLSPMV_ERR_INVALID, "panel encoding needs row-sorted entries");
+        blk_row.push_back(rows);
+    }
+    const int nb = (int)blk_row.size() - 1;
+    if ((long long)P * (nb + 1) > (1ll << 26))
+        return fail(CLSPMV_ERR_UNSUPPORTED, "%d panels x %d row blocks: too many groups", P, nb);
+    std::vector<int> blk_of(rows);
+    for (int b = 0; b < nb; ++b)
+        std::fill(blk_of.begin() + blk_row[b], blk_of.begin() + blk_row[b + 1], b);
+
+    // Groups (panel, block) in pass-1 order, each padded to a multiple of 4
+    // entries (one lane's load) and cut into sub-chunks of kSub entries, the
+    // last one shorter.
+    const size_t G = (size_t)P * nb;
+    std::vector<long long> cnt(G, 0);
+    for (long long i = 0; i < nnz; ++i) cnt[(size_t)(col[i] / pw) * nb + blk_of[row[i]]]++;
+    std::vector<long long> gstart(G + 1, 0);
+    for (size_t g = 0; g < G; ++g) gstart[g + 1] = gstart[g] + (cnt[g] + 3) / 4 * 4;
+    const long long np = gstart[G];
     if (np >= INT_MAX)
         return fail(CLSPMV_ERR_TOO_LARGE, "panel encoding of %lld entries exceeds 32-bit indices", np);
+    std::vector<int> sc_start;   // first entry of each sub-chunk (+ end)
+    std::vector<int> g_sc(G + 1);  // first sub-chunk of each group
+    for (size_t g = 0; g < G; ++g) {
+        g_sc[g] = (int)sc_start.size();
+        for (long long k = gstart[g]; k < gstart[g + 1]; k += kSub) sc_start.push_back((int)k);
+    }
+    g_sc[G] = (int)sc_start.size();
+    sc_start.push_back((int)np);
+    const long long nsc = (long long)sc_start.size() - 1;
 
-    // Stable regrouping by panel keeps every panel row-sorted.  Padding
-    // entries (row -1) fill each panel to whole sub-chunks.
-    std::vector<unsigned short> pc(np, 0);
+    // Stable scatter keeps each group row-sorted; padding: column code
+    // kPanelPad, value 0, row -1.
+    std::vector<unsigned short> pc(np, (unsigned short)launch::kPanelPad);
     std::vector<float> pv(np, 0.0f);
     std::vector<int> prow(np, -1);
-    std::vector<long long> cur(start.begin(), start.end() - 1);
-    for (long long i = 0; i < nnz; ++i) {
-        const int p = col[i] / pw;
-        const long long k = cur[p]++;
-        pc[k] = (unsigned short)(col[i] - p * pw);
-        pv[k] = val[i];
-        prow[k] = row[i];
+    {
+        std::vector<long long> cur(gstart.begin(), gstart.end() - 1);
+        for (long long i = 0; i < nnz; ++i) {
+            const int p = col[i] / pw;
+            const long long k = cur[(size_t)p * nb + blk_of[row[i]]]++;
+            pc[k] = (unsigned short)(col[i] - p * pw);
+            pv[k] = val[i];
+            prow[k] = row[i];
+        }
     }
 
-    // Segments: a new one at every sub-chunk start and at every row change.
-    const long long nsc = np / kSub;
+    // Segments: one at every sub-chunk start and at every row change.
+    // Padding (under 4 entries, at a group's end) never starts a sub-chunk,
+    // so every segment has a row.  Count per group first.
+    std::vector<int> gseg(G, 0);
+    for (size_t g = 0; g < G; ++g)
+        for (long long k = gstart[g]; k < gstart[g + 1]; ++k) {
+            const bool flag = ((k - gstart[g]) % kSub == 0) || (prow[k] >= 0 && prow[k] != prow[k - 1]);
+            if (flag) {
+                pc[k] |= 0x8000u;
+                gseg[g]++;
+            }
+        }
+    // Numbering in (block, panel) order: the first id of each group.
+    std::vector<int> blk_tab((size_t)nb * (P + 1));
+    std::vector<long long> gbase(G);
+    long long nseg = 0;
+    for (int b = 0; b < nb; ++b) {
+        for (int p = 0; p < P; ++p) {
+            blk_tab[(size_t)b * (P + 1) + p] = (int)nseg;
+            gbase[(size_t)p * nb + b] = nseg;
+            nseg += gseg[(size_t)p * nb + b];
+        }
+        blk_tab[(size_t)b * (P + 1) + P] = (int)nseg;
+    }
+    if (nseg >= INT_MAX) return fail(CLSPMV_ERR_TOO_LARGE, "too many segments");
     std::vector<int> sc_seg(nsc);
-    std::vector<int> seg_row;
-    seg_row.reserve(nsc + (size_t)rows);
-    for (long long k = 0; k < np; ++k) {
-        bool flag;
-        if (k % kSub == 0) {
-            sc_seg[k / kSub] = (int)seg_row.size();
-            flag = true;
-        } else {
-            flag = prow[k] >= 0 && prow[k] != prow[k - 1];
-        }
-        if (flag) {
-            seg_row.push_back(prow[k]);
-            pc[k] |= 0x8000u;
+    std::vector<int> seg_row(nseg);
+    for (size_t g = 0; g < G; ++g) {
+        long long id = gbase[g] - 1;
+        int sc = g_sc[g];
+        for (long long k = gstart[g]; k < gstart[g + 1]; ++k) {
+            if (sc < g_sc[g + 1] && k == sc_start[sc]) sc_seg[sc++] = (int)(id + 1);
+            if (pc[k] & 0x8000u) {
+                ++id;
+                // Bit 31: the segment continues its predecessor's row (a run
+                // cut at a sub-chunk boundary); bit 30 on the predecessor:
+                // a continuation follows.
+                const bool cont = k > gstart[g] && prow[k] == prow[k - 1];
+                seg_row[id] = prow[k] | (cont ? (int)0x80000000u : 0);
+                if (cont) seg_row[id - 1] |= 0x40000000;
+            }
         }
     }
-    const long long nseg = (long long)seg_row.size();
 
-    // Per-row segment lists, rows ascending, segments in panel order.
-    std::vector<int> per_row(rows, 0);
-    for (int r : seg_row)
-        if (r >= 0) per_row[r]++;
-    std::vector<int> rrow, rs_ptr(1, 0), slot(rows, -1);
-    for (int r = 0; r < rows; ++r)
-        if (per_row[r]) {
-            slot[r] = rs_ptr.back();
-            rrow.push_back(r);
-            rs_ptr.push_back(rs_ptr.back() + per_row[r]);
+    // What the kernels rely on: every sub-chunk opens a segment, its
+    // segments' slots exist, and its lanes read whole 4-entry groups; each
+    // block's segments fit one pass-2 work-group and belong to its rows.
+    for (long long c = 0; c < nsc; ++c) {
+        const int k0 = sc_start[c], k1 = sc_start[c + 1];
+        if (!(pc[k0] & 0x8000u) || k1 - k0 > kSub || (k1 - k0) % 4)
+            return fail(CLSPMV_ERR_INVALID, "panel encoding: malformed sub-chunk %lld", c);
+        int nf = 0;
+        for (int k = k0; k < k1; ++k) nf += (pc[k] >> 15) & 1;
+        if ((long long)sc_seg[c] + nf > nseg)
+            return fail(CLSPMV_ERR_INVALID, "panel encoding: sub-chunk %lld writes past the segments", c);
+    }
+    for (int b = 0; b < nb; ++b) {
+        const int j0 = blk_tab[(size_t)b * (P + 1)], j1 = blk_tab[(size_t)b * (P + 1) + P];
+        if (j1 - j0 > launch::kBlockSegsMax || blk_row[b + 1] - blk_row[b] > launch::kBlockRowsMax)
+            return fail(CLSPMV_ERR_INVALID, "panel encoding: block %d over its limits", b);
+        for (int j = j0; j < j1; ++j) {
+            const int r = seg_row[j] & 0x3FFFFFFF;
+            if (r < blk_row[b] || r >= blk_row[b + 1])
+                return fail(CLSPMV_ERR_INVALID, "panel encoding: segment %d outside block %d", j, b);
         }
-    std::vector<int> rs_idx(rs_ptr.back());
-    for (long long s = 0; s < nseg; ++s)
-        if (seg_row[s] >= 0) rs_idx[slot[seg_row[s]]++] = (int)s;
+    }
 
-    // Work-groups: runs of at most G sub-chunks of one panel.
-    const int G = std::max(32, std::min(128, pw / kSub));
+    // Pass-1 work-groups: runs of at most W sub-chunks of one panel.
+    const int W = std::max(32, std::min(128, pw / kSub));
     std::vector<int> wg;
     for (int p = 0; p < P; ++p) {
-        const int b0 = (int)(start[p] / kSub), b1 = (int)(start[p + 1] / kSub);
-        const int real_end = (int)(start[p] + cnt[p]);
-        for (int b = b0; b < b1; b += G) {
+        const int b0 = g_sc[(size_t)p * nb], b1 = g_sc[(size_t)(p + 1) * nb];
+        for (int b = b0; b < b1; b += W) {
             wg.push_back(p);
             wg.push_back(b);
-            wg.push_back(std::min(b + G, b1));
-            wg.push_back(real_end);
+            wg.push_back(std::min(b + W, b1));
+            wg.push_back(0);
         }
     }
 
     d->pw = pw;
     d->pnwg = (int)(wg.size() / 4);
-    d->prows = (int)rrow.size();
+    d->pnp = P;
+    d->pnb = nb;
     int st = upload_array(d, wg.data(), wg.size(), &d->pwg);
     if (!st) st = upload_array(d, pc.data(), pc.size(), &d->pcol);
     if (!st) st = upload_array(d, pv.data(), pv.size(), &d->pval);
     if (!st) st = upload_array(d, sc_seg.data(), sc_seg.size(), &d->psc_seg);
+    if (!st) st = upload_array(d, sc_start.data(), sc_start.size(), &d->psc_start);
     if (!st) st = upload_array<float>(d, nullptr, (size_t)nseg, &d->pseg);
-    if (!st) st = upload_array(d, rrow.data(), rrow.size(), &d->prrow);
-    if (!st) st = upload_array(d, rs_ptr.data(), rs_ptr.size(), &d->prs_ptr);
-    if (!st) st = upload_array(d, rs_idx.data(), rs_idx.size(), &d->prs_idx);
+    if (!st) st = upload_array(d, seg_row.data(), seg_row.size(), &d->pseg_row);
+    if (!st) st = upload_array(d, blk_row.data(), blk_row.size(), &d->pblk_row);
+    if (!st) st = upload_array(d, blk_tab.data(), blk_tab.size(), &d->pblk_tab);
     if (st) return st;
     d->use_panel = true;
     return CLSPMV_OK;
@@ -122,10 +210,11 @@ void drop_encoding(clspmv_dmat* d, bool panel) {
         release_array(d, &d->pcol);
         release_array(d, &d->pval);
         release_array(d, &d->psc_seg);
+        release_array(d, &d->psc_start);
         release_array(d, &d->pseg);
-        release_array(d, &d->prrow);
-        release_array(d, &d->prs_ptr);
-        release_array(d, &d->prs_idx);
+        release_array(d, &d->pseg_row);
+        release_array(d, &d->pblk_row);
+        release_array(d, &d->pblk_tab);
         d->pw = 0;
         d->use_panel = false;
         return;

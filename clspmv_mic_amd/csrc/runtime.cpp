@@ -107,8 +107,9 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
     const bool xa = aligned16(x);
     if ((I.format == CLSPMV_FMT_CSR || I.format == CLSPMV_FMT_COO) && A->use_panel) {
         // The panel encoding serves every method of the format.
-        e = launch::panel_spmv(s, A->pnwg, A->pwg, A->pw, I.cols, A->pcol, A->pval, A->psc_seg, x, xa,
-                               A->pseg, A->prows, A->prrow, A->prs_ptr, A->prs_idx, y);
+        e = launch::panel_spmv(s, A->pnwg, A->pwg, A->pw, I.cols, A->pcol, A->pval, A->psc_seg,
+                               A->psc_start, x, xa,
+                               A->pseg, A->pnp, A->pnb, A->pblk_row, A->pblk_tab, A->pseg_row, y);
         if (e != hipSuccess)
             return fail(CLSPMV_ERR_HIP, "panel launch failed (format %d): %s", I.format,
                         hipGetErrorString(e));
@@ -325,6 +326,7 @@ static int add_panel(clspmv_dmat* d, int panel_width, const int* row, const int*
     if (pw == CLSPMV_PANEL_AUTO || pw == CLSPMV_PANEL_BOTH) pw = auto_panel_width(I.cols, I.nnz);
     if (pw == 0) return CLSPMV_OK;
     int st = build_panel(d, I.rows, I.cols, I.nnz, row, col, val, pw);
+    if (st == CLSPMV_ERR_UNSUPPORTED && panel_width < 0) return CLSPMV_OK;  // stay plain
     if (st) return st;
     if (panel_width != CLSPMV_PANEL_BOTH) drop_encoding(d, false);
     return CLSPMV_OK;

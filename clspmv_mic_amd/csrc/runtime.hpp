@@ -44,15 +44,16 @@ struct clspmv_dmat {
     int brows = 0, aligned = 0, f4 = 0;
     // x-panel encoding of COO / CSR (pw == 0: absent).  plain == false when
     // the (row, col, val) / CSR arrays were not uploaded.
-    int pw = 0, pnwg = 0, prows = 0;
-    int* pwg = nullptr;  // int4 per work-group
+    int pw = 0, pnwg = 0, pnp = 0, pnb = 0;
+    int* pwg = nullptr;  // int4 per pass-1 work-group
     unsigned short* pcol = nullptr;
     float* pval = nullptr;
     int* psc_seg = nullptr;
+    int* psc_start = nullptr;  // first entry of each sub-chunk (+ end)
     float* pseg = nullptr;
-    int* prrow = nullptr;
-    int* prs_ptr = nullptr;
-    int* prs_idx = nullptr;
+    int* pseg_row = nullptr;   // row | continuation bit, per segment
+    int* pblk_row = nullptr;   // first row of each row block (+ end)
+    int* pblk_tab = nullptr;   // first segment of (block, panel)
     bool use_panel = false;
     bool plain = true;
 };
