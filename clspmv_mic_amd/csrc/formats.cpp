@@ -875,9 +875,23 @@ int clspmv_read_mmf(const char* filename, clspmv_coo_matrix* dst) {
             std::fclose(f);
             return fail(CLSPMV_ERR_IO, "read_mmf: truncated banner");
         }
-    // Field 4 is the value type, field 5 the symmetry (fileio.cpp:17-24).
-    bool has_values = std::strcmp(banner[3], "pattern") != 0;
-    bool symmetric = std::strcmp(banner[4], "symmetric") == 0;
+    // Banner: %%MatrixMarket matrix coordinate <field> <symmetry>; field 4
+    // is the value type, field 5 the symmetry (src/fileio.cpp:15-25).  The
+    // reference parses values only for "real" and mirrors only "symmetric";
+    // here integer values are parsed too, skew-symmetric matrices are
+    // mirrored with negated values, and complex or dense (array) files are
+    // refused instead of misread.
+    if (std::strcmp(banner[2], "coordinate") != 0) {
+        std::fclose(f);
+        return fail(CLSPMV_ERR_UNSUPPORTED, "read_mmf: only coordinate files are supported (got %s)", banner[2]);
+    }
+    if (std::strcmp(banner[3], "complex") == 0) {
+        std::fclose(f);
+        return fail(CLSPMV_ERR_UNSUPPORTED, "read_mmf: complex values are not supported");
+    }
+    const bool has_values = std::strcmp(banner[3], "pattern") != 0;
+    const bool skew = std::strcmp(banner[4], "skew-symmetric") == 0;
+    const bool symmetric = skew || std::strcmp(banner[4], "symmetric") == 0;
     char line[4096];
     int height = 0, width = 0, nnz = 0;
     while (std::fgets(line, sizeof line, f)) {
@@ -903,7 +917,7 @@ int clspmv_read_mmf(const char* filename, clspmv_coo_matrix* dst) {
         }
         r.push_back(ri - 1); c.push_back(ci - 1); v.push_back((float)val);
         if (symmetric && ri != ci) {
-            r.push_back(ci - 1); c.push_back(ri - 1); v.push_back((float)val);
+            r.push_back(ci - 1); c.push_back(ri - 1); v.push_back(skew ? -(float)val : (float)val);
         }
     }
     std::fclose(f);
@@ -914,7 +928,10 @@ int clspmv_read_mmf(const char* filename, clspmv_coo_matrix* dst) {
     dst->coo_col_id = host_copy(c);
     dst->coo_data = host_copy(v);
     int st = clspmv_validate_coo(dst);
-    if (st) return st;
+    if (st) {
+        clspmv_free_coo_matrix(dst);
+        return st;
+    }
     if (!coo_sorted(*dst)) coo_sort(*dst);
     return CLSPMV_OK;
 }

@@ -268,7 +268,12 @@ int clspmv_bcsr2coo(const clspmv_bcsr_matrix* src, clspmv_coo_matrix* dst);   /*
 int clspmv_bell2coo(const clspmv_bell_matrix* src, clspmv_coo_matrix* dst);   /* :1901 */
 
 /* ---- Matrix Market ingest (src/fileio.cpp:10) ------------------------- */
-/* ReadMMF: 1-based -> 0-based, pattern -> 1.0, symmetric expansion, sorted. */
+/* ReadMMF: coordinate files; 1-based -> 0-based, pattern -> 1.0, values
+ * read as double and rounded to float, symmetric (and skew-symmetric, with
+ * negated mirror values) expansion without doubling the diagonal, result
+ * sorted by (row, col), duplicates kept.  Complex and array files are
+ * refused (CLSPMV_ERR_UNSUPPORTED); indices outside the stated size give
+ * CLSPMV_ERR_INVALID.  dst's arrays are malloc()ed (clspmv_free_coo_matrix). */
 int clspmv_read_mmf(const char* filename, clspmv_coo_matrix* dst);
 
 #ifdef __cplusplus

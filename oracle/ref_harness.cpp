@@ -1,6 +1,6 @@
 // ref_harness.cpp -- C entry points into the reference's own CPU code
 // (include/matrix_storage.h, include/spmv_serial.h, src/analyze.cpp,
-// src/util.cpp), compiled from the sources under /root/reference into
+// src/util.cpp, src/fileio.cpp), compiled from the sources under /root/reference into
 // oracle/_ref/libclspmv_ref.so by oracle/Makefile.  TEST INFRASTRUCTURE
 // ONLY: used to pin the oracle and to write golden fixtures; the engine
 // never links it.
@@ -15,6 +15,7 @@
 #include <unistd.h>
 
 #include "analyze.h"          // reference
+#include "fileio.h"           // reference
 #include "matrix_storage.h"   // reference
 #include "spmv_serial.h"      // reference
 #include "util.h"             // reference
@@ -129,6 +130,8 @@ void ref_bcsr_spmv(clspmv_bcsr_matrix* m, float* x, float* y) { bcsr_spmv<int, f
 void ref_bell_spmv(clspmv_bell_matrix* m, float* x, float* y) { bell_spmv<int, float>(R<rbell>(m), x, y, m->matinfo.width); }
 void ref_sbell_spmv(clspmv_sbell_matrix* m, float* x, float* y) { sbell_spmv<int, float>(R<rsbell>(m), x, y, m->matinfo.width); }
 void ref_spmv_only(clspmv_coo_matrix* m, float* x, float* y) { spmv_only(R<rcoo>(m), x, y); }
+// ReadMMF (src/fileio.cpp:10); arrays malloc()ed, release with free().
+void ref_read_mmf(const char* path, clspmv_coo_matrix* m) { ReadMMF(const_cast<char*>(path), R<rcoo>(m)); }
 
 // analyze_matrix with the tables under <ben_dir>/benchmark (the reference
 // reads $CLSPMVPATH, src/analyze.cpp:179).
