@@ -236,6 +236,10 @@ _sig("clspmv_dmat_get_info", C.c_int, VP, P(DmatInfo))
 _sig("clspmv_upload_coo_ex", C.c_int, VP, P(CooMatrix), C.c_int, P(VP))
 _sig("clspmv_upload_csr_ex", C.c_int, VP, P(CsrMatrix), C.c_int, P(VP))
 _sig("clspmv_dmat_set_panel", C.c_int, VP, C.c_int)
+_sig("clspmv_upload_bcsr_lowp", C.c_int, VP, P(BcsrMatrix), C.c_int, P(VP))
+_sig("clspmv_spmm", C.c_int, VP, VP, C.c_int, VP, C.c_int, VP, C.c_int)
+_sig("clspmv_convert_from_f32", C.c_int, PF, VP, C.c_size_t, C.c_int)
+_sig("clspmv_convert_to_f32", C.c_int, VP, PF, C.c_size_t, C.c_int)
 _sig("clspmv_spmv", C.c_int, VP, VP, C.c_int, VP, VP)
 _sig("clspmv_time_spmv", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int, C.c_int, P(C.c_double))
 _sig("clspmv_bandwidth_probe", C.c_int, VP, C.c_size_t, C.c_int, P(C.c_double), P(C.c_double))
@@ -574,6 +578,71 @@ _UPLOAD = {CsrMatrix: "csr", CooMatrix: "coo", EllMatrix: "ell", SellMatrix: "se
 
 PANEL_AUTO = -1
 PANEL_BOTH = -2
+DT_F32, DT_BF16, DT_F16 = 0, 1, 2
+FMT_BSPMM = 9
+
+
+def convert_from_f32(a: np.ndarray, dtype: int) -> np.ndarray:
+    """float32 -> 16-bit storage (round to nearest even), as uint16 bits."""
+    a = np.ascontiguousarray(a, np.float32)
+    out = np.empty(a.shape, np.uint16)
+    check(lib.clspmv_convert_from_f32(_fp(a.reshape(-1)), out.ctypes.data_as(VP), a.size, dtype),
+          "convert_from_f32")
+    return out
+
+
+def convert_to_f32(bits: np.ndarray, dtype: int) -> np.ndarray:
+    bits = np.ascontiguousarray(bits, np.uint16)
+    out = np.empty(bits.shape, np.float32)
+    check(lib.clspmv_convert_to_f32(bits.ctypes.data_as(VP), _fp(out.reshape(-1)), bits.size, dtype),
+          "convert_to_f32")
+    return out
+
+
+class DeviceBuffer:
+    """Raw device bytes (16-bit operands of clspmv_spmm)."""
+
+    def __init__(self, ctx: Context, host: np.ndarray):
+        h = np.ascontiguousarray(host)
+        self.ctx, self.nbytes, self.dtype, self.shape = ctx, h.nbytes, h.dtype, h.shape
+        p = VP()
+        check(lib.clspmv_device_malloc(ctx.h, max(self.nbytes, 16), C.byref(p)), "device_malloc")
+        self.p = p
+        check(lib.clspmv_memcpy_h2d(ctx.h, p, h.ctypes.data_as(VP), self.nbytes), "h2d")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(lib.clspmv_memcpy_d2h(self.ctx.h, out.ctypes.data_as(VP), self.p, self.nbytes), "d2h")
+        return out
+
+    def __del__(self):
+        try:
+            if self.p:
+                lib.clspmv_device_free(self.ctx.h, self.p)
+        except Exception:
+            pass
+
+
+class BlockSpmm:
+    """A 4x4-block BCSR matrix with BF16 / FP16 values on the device and its
+    MFMA SpMM: Y += A X (clspmv_upload_bcsr_lowp / clspmv_spmm)."""
+
+    def __init__(self, ctx: Context, bcsr: "Owned", dtype: int):
+        h = VP()
+        check(lib.clspmv_upload_bcsr_lowp(ctx.h, C.byref(bcsr.s), dtype, C.byref(h)), "upload_bcsr_lowp")
+        self.ctx, self.h, self.dtype = ctx, h, dtype
+        self.info = DmatInfo()
+        check(lib.clspmv_dmat_get_info(h, C.byref(self.info)), "dmat_info")
+
+    def spmm(self, k: int, X, ldx: int, Y, ldy: int):
+        check(lib.clspmv_spmm(self.ctx.h, self.h, k, X.p, ldx, Y.p, ldy), "spmm")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.clspmv_dmat_free(self.h)
+        except Exception:
+            pass
 
 
 class DeviceMatrix:

@@ -90,6 +90,15 @@ hipError_t sbell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
                  int slice_height, const int* sptr, const int* col4,
                  const float* data, const float* x, bool x_aligned, float* y);
 
+// ---- mixed-precision 4x4-block SpMM (MFMA) -------------------------------------
+// Y[:, 0:k] += A X[:, 0:k]; A: block rows rp/bcol (element columns, multiples
+// of 4) with 16 BF16 / FP16 values per block, row-major; X: 16-bit, column-
+// major (ldx); Y: FP32 column-major (ldy).  ldx, ldy multiples of 4 and X, Y
+// 16-byte aligned.
+hipError_t bspmm4(hipStream_t s, bool bf16, int brows, int rows, int cols, int k,
+                  const int* rp, const int* bcol, const unsigned short* val,
+                  const unsigned short* X, int ldx, float* Y, int ldy);
+
 // ---- bandwidth probe ---------------------------------------------------------
 // dst[0..4*n4) = src[0..4*n4) with 128-bit loads/stores (16-byte aligned).
 hipError_t copy4(hipStream_t s, long long n4, const float* src, float* dst);

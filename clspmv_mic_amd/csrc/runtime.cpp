@@ -9,6 +9,7 @@
 // indices are range-checked on upload so that no kernel can read outside x.
 
 #include <algorithm>
+#include <cstring>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -285,6 +286,7 @@ int clspmv_memset_zero(clspmv_ctx* ctx, void* dst, size_t bytes) {
 
 int clspmv_method_count(int format) {
     switch (format) {
+        case CLSPMV_FMT_BSPMM: return 0;  // clspmv_spmm only
         case CLSPMV_FMT_CSR: return CLSPMV_CSR_IMP_NUM;
         case CLSPMV_FMT_COO: return CLSPMV_COO_IMP_NUM;
         case CLSPMV_FMT_ELL: return CLSPMV_ELL_IMP_NUM;
@@ -591,8 +593,6 @@ int clspmv_upload_sbell(clspmv_ctx* ctx, const clspmv_sbell_matrix* m, clspmv_dm
     return CLSPMV_OK;
 }
 
-#undef UPLOAD_PROLOGUE
-#undef UPLOAD_CHECK
 
 int clspmv_dmat_free(clspmv_dmat* m) {
     if (!m) return CLSPMV_OK;
@@ -602,6 +602,132 @@ int clspmv_dmat_free(clspmv_dmat* m) {
     for (auto& b : m->bufs) (void)hipFree(b.first);
     if (cur != m->device && cur >= 0) (void)hipSetDevice(cur);
     delete m;
+    return CLSPMV_OK;
+}
+
+// ---- mixed-precision block SpMM ------------------------------------------------
+
+namespace {
+
+uint16_t f32_to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu))
+        return (uint16_t)((u >> 16) | 0x0040u);  // quiet NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);            // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+float bf16_to_f32(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+uint16_t f32_to_f16(float f) {
+    const _Float16 h = (_Float16)f;  // round to nearest even
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
+float f16_to_f32(uint16_t u) {
+    _Float16 h;
+    std::memcpy(&h, &u, 2);
+    return (float)h;
+}
+
+}  // namespace
+
+int clspmv_convert_from_f32(const float* src, void* dst, size_t n, int dtype) {
+    if ((!src || !dst) && n) return fail(CLSPMV_ERR_INVALID, "null argument");
+    uint16_t* d = static_cast<uint16_t*>(dst);
+    if (dtype == CLSPMV_DT_BF16) {
+        for (size_t i = 0; i < n; ++i) d[i] = f32_to_bf16(src[i]);
+    } else if (dtype == CLSPMV_DT_F16) {
+        for (size_t i = 0; i < n; ++i) d[i] = f32_to_f16(src[i]);
+    } else {
+        return fail(CLSPMV_ERR_INVALID, "dtype %d is not a 16-bit type", dtype);
+    }
+    return CLSPMV_OK;
+}
+
+int clspmv_convert_to_f32(const void* src, float* dst, size_t n, int dtype) {
+    if ((!src || !dst) && n) return fail(CLSPMV_ERR_INVALID, "null argument");
+    const uint16_t* s = static_cast<const uint16_t*>(src);
+    if (dtype == CLSPMV_DT_BF16) {
+        for (size_t i = 0; i < n; ++i) dst[i] = bf16_to_f32(s[i]);
+    } else if (dtype == CLSPMV_DT_F16) {
+        for (size_t i = 0; i < n; ++i) dst[i] = f16_to_f32(s[i]);
+    } else {
+        return fail(CLSPMV_ERR_INVALID, "dtype %d is not a 16-bit type", dtype);
+    }
+    return CLSPMV_OK;
+}
+
+int clspmv_upload_bcsr_lowp(clspmv_ctx* ctx, const clspmv_bcsr_matrix* m, int dtype,
+                            clspmv_dmat** out) {
+    if (dtype != CLSPMV_DT_BF16 && dtype != CLSPMV_DT_F16)
+        return fail(CLSPMV_ERR_INVALID, "dtype %d: need CLSPMV_DT_BF16 or CLSPMV_DT_F16", dtype);
+    UPLOAD_PROLOGUE(m, CLSPMV_FMT_BSPMM)
+    if (m->bcsr_bwidth != 4 || m->bcsr_bheight != 4)
+        UPLOAD_CHECK(fail(CLSPMV_ERR_UNSUPPORTED, "block SpMM needs 4x4 blocks (got %ux%u)",
+                          m->bcsr_bheight, m->bcsr_bwidth));
+    const int nbr = m->bcsr_row_num, nb = m->bcsr_block_num, rows = m->matinfo.height,
+              cols = m->matinfo.width;
+    if (nbr < 0 || nb < 0 || (long long)nbr * 4 < rows || m->bcsr_aligned_size < nb ||
+        (nbr > 0 && (!m->bcsr_row_ptr || m->bcsr_row_ptr[0] != 0 || m->bcsr_row_ptr[nbr] != nb)))
+        UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed BCSR matrix"));
+    for (int r = 0; r < nbr; ++r)
+        if (m->bcsr_row_ptr[r + 1] < m->bcsr_row_ptr[r])
+            UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "BCSR row pointer decreases at %d", r));
+    for (int j = 0; j < nb; ++j) {
+        const int c = m->bcsr_col_id[j];
+        if (c < 0 || c >= cols || c % 4)
+            UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "BCSR block %d: column %d", j, c));
+    }
+    // Element (h, w) of block j sits at data[j + (4h + w) * aligned]
+    // (include/matrix_storage.h:1719-1735); the device keeps each block's 16
+    // values together, row by row.
+    std::vector<float> blocks((size_t)nb * 16);
+    for (int j = 0; j < nb; ++j)
+        for (int e = 0; e < 16; ++e)
+            blocks[(size_t)j * 16 + e] = m->bcsr_data[j + (size_t)e * m->bcsr_aligned_size];
+    std::vector<uint16_t> lp(blocks.size());
+    UPLOAD_CHECK(clspmv_convert_from_f32(blocks.data(), lp.data(), lp.size(), dtype));
+    d->dtype = dtype;
+    d->brows = nbr;
+    d->info.bheight = d->info.bwidth = 4;
+    d->info.stored = (long long)nb * 16;
+    UPLOAD_CHECK(upload_array(d, m->bcsr_row_ptr, (size_t)nbr + 1, &d->rp));
+    UPLOAD_CHECK(upload_array(d, m->bcsr_col_id, (size_t)nb, &d->col));
+    UPLOAD_CHECK(upload_array(d, lp.data(), lp.size(), &d->lpval));
+    *out = d;
+    return CLSPMV_OK;
+}
+
+#undef UPLOAD_PROLOGUE
+#undef UPLOAD_CHECK
+
+int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, int ldx,
+                float* d_Y, int ldy) {
+    if (!ctx || !A || (!d_X && k) || (!d_Y && k)) return fail(CLSPMV_ERR_INVALID, "null argument");
+    const clspmv_dmat_info& I = A->info;
+    if (I.format != CLSPMV_FMT_BSPMM)
+        return fail(CLSPMV_ERR_UNSUPPORTED, "clspmv_spmm needs a matrix from clspmv_upload_bcsr_lowp");
+    if (k < 0) return fail(CLSPMV_ERR_INVALID, "k = %d", k);
+    if (ldx < I.cols || ldx % 4 || ldy < I.rows || ldy % 4)
+        return fail(CLSPMV_ERR_INVALID, "ldx %d / ldy %d: need >= cols / rows and multiples of 4", ldx, ldy);
+    if (((uintptr_t)d_X & 7) || ((uintptr_t)d_Y & 15))
+        return fail(CLSPMV_ERR_INVALID, "X must be 8-byte and Y 16-byte aligned");
+    int st = use_ctx_device(ctx);
+    if (st) return st;
+    if (k == 0 || I.rows == 0) return CLSPMV_OK;
+    hipError_t e = launch::bspmm4(ctx->stream, A->dtype == CLSPMV_DT_BF16, A->brows, I.rows, I.cols, k,
+                                  A->rp, A->col, A->lpval, static_cast<const unsigned short*>(d_X), ldx,
+                                  d_Y, ldy);
+    if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "spmm launch failed: %s", hipGetErrorString(e));
     return CLSPMV_OK;
 }
 

@@ -56,6 +56,10 @@ struct clspmv_dmat {
     int* pblk_tab = nullptr;   // first segment of (block, panel)
     bool use_panel = false;
     bool plain = true;
+    // BSPMM: 16 BF16 / FP16 values per 4x4 block, row-major; rp / col hold
+    // the block rows and element columns.
+    unsigned short* lpval = nullptr;
+    int dtype = 0;
 };
 
 namespace clspmv {

@@ -49,7 +49,8 @@ enum clspmv_format {
     CLSPMV_FMT_BDIA  = 5,   /* bdia_matrix  -> kernels/spmv_bdia.cl    */
     CLSPMV_FMT_BCSR  = 6,   /* b4csr_matrix -> kernels/spmv_bcsr.cl    */
     CLSPMV_FMT_BELL  = 7,   /* b4ell_matrix -> kernels/spmv_bell.cl    */
-    CLSPMV_FMT_SBELL = 8    /* sbell_matrix -> kernels/spmv_sbell.cl   */
+    CLSPMV_FMT_SBELL = 8,   /* sbell_matrix -> kernels/spmv_sbell.cl   */
+    CLSPMV_FMT_BSPMM = 9    /* 4x4-block BCSR, 16-bit values: SpMM only */
 };
 
 /* Kernel variants ("methods") per format.  Method numbers index the
@@ -139,6 +140,32 @@ int clspmv_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
 int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
                      const float* d_x, float* d_y, int warmup, int ntimes,
                      double* ms_per_call);
+
+/* ---- mixed-precision block SpMM (no counterpart in the reference) -------
+ * Y += A X for k right-hand sides with A's values stored in BF16 or FP16
+ * and the products accumulated in FP32 on the matrix cores (MFMA 4x4x4: one
+ * 4x4 block of A times a 4x4 tile of X per group of four lanes).
+ *
+ * clspmv_upload_bcsr_lowp: A from a bcsr_matrix with 4x4 blocks
+ *   (include/matrix_storage.h:178, from clspmv_coo2bcsr(..., 4, 4, ...)),
+ *   values rounded to `dtype` (round to nearest even).  The result has
+ *   format CLSPMV_FMT_BSPMM and runs only through clspmv_spmm.
+ * clspmv_spmm: X holds cols x k values of the same dtype, column-major with
+ *   leading dimension ldx; Y holds rows x k FP32 values, column-major with
+ *   leading dimension ldy.  ldx >= cols and ldy >= rows, both multiples of
+ *   4; X 8-byte and Y 16-byte aligned.  Asynchronous on the context's
+ *   stream.  Each Y element is written by one lane: no atomics, a fixed
+ *   summation order.
+ * clspmv_convert_from_f32 / clspmv_convert_to_f32: host conversions of n
+ *   values between float and dtype (round to nearest even), for building X
+ *   and for exact references. */
+enum clspmv_dtype { CLSPMV_DT_F32 = 0, CLSPMV_DT_BF16 = 1, CLSPMV_DT_F16 = 2 };
+int clspmv_upload_bcsr_lowp(clspmv_ctx* ctx, const clspmv_bcsr_matrix* m, int dtype,
+                            clspmv_dmat** out);
+int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, int ldx,
+                float* d_Y, int ldy);
+int clspmv_convert_from_f32(const float* src, void* dst, size_t n, int dtype);
+int clspmv_convert_to_f32(const void* src, float* dst, size_t n, int dtype);
 
 /* ---- bandwidth probe (src/mem_bandwidth.cpp bandwidth_test :14) ---------
  * Copies `bytes` (rounded down to 16) between two fresh device buffers

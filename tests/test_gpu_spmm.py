@@ -1,0 +1,93 @@
+"""GPU parity of the mixed-precision MFMA block SpMM (clspmv_spmm): Y += A X
+with A's 4x4 blocks and X in BF16 or FP16, accumulated in FP32.
+
+Reference: the FP64 oracle applied to the values as stored (A and X rounded
+to 16 bits, exactly representable in FP32), one right-hand side at a time;
+bar |Y - Y64| <= 1e-5 (|Y0| + |A||X|) per element, and bit-exact on dyadic
+inputs, where every product and partial sum is exact.  The rounding error
+of storing A and X in 16 bits is reported by bench.py, not bounded here."""
+
+import numpy as np
+import pytest
+
+import clspmv_mic_amd as cs
+from clspmv_mic_amd import matrices
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _cases():
+    return {
+        "block_small": matrices.block_stencil(700, 4, 4, seed=6, offsets=(0, 1, -1, 32, -33, 5)),
+        "rand": matrices.random_sparse(501, 333, 0.02, seed=41),       # rows/cols not multiples of 4
+        "rand_dyadic": matrices.random_sparse(300, 260, 0.05, seed=42, dyadic=True),
+        "wide": matrices.random_sparse(37, 1201, 0.03, seed=43),
+        "long_rows": matrices.random_sparse(64, 3000, 0.002, seed=44, long_rows=3, long_len=2500),
+    }
+
+
+CASES = _cases()
+
+
+def _run(ctx, oracle, A, dtype, k, pad=0):
+    dyadic = bool(np.all(A.val * 64 == np.round(A.val * 64)))
+    rng = np.random.default_rng(k * 7 + dtype)
+    H, W = A.height, A.width
+    ldx = (W + 3) // 4 * 4 + pad
+    ldy = (H + 3) // 4 * 4 + pad
+    if dyadic:
+        X = (rng.integers(-64, 65, size=(k, ldx)) / 64.0).astype(np.float32)
+        Y0 = (rng.integers(-64, 65, size=(k, ldy)) / 64.0).astype(np.float32)
+    else:
+        X = rng.uniform(-1, 1, size=(k, ldx)).astype(np.float32)
+        Y0 = rng.uniform(-1, 1, size=(k, ldy)).astype(np.float32)
+    Xb = cs.convert_from_f32(X, dtype)               # column j of X = row j here
+    Xs = cs.convert_to_f32(Xb, dtype)                # the values the GPU sees
+    vals_s = cs.convert_to_f32(cs.convert_from_f32(A.val, dtype), dtype)
+    bc = cs.coo2bcsr(A.copy(), 4, 4)
+    M = cs.BlockSpmm(ctx, bc, dtype)
+    assert M.info.format == cs.FMT_BSPMM
+    dX = cs.DeviceBuffer(ctx, Xb)
+    dY = cs.DeviceBuffer(ctx, Y0)
+    M.spmm(k, dX, ldx, dY, ldy)
+    ctx.sync()
+    Y = dY.download()
+    for j in range(k):
+        y64, bound = oracle.spmv_f64(A.row, A.col, vals_s, Xs[j, :W].copy(), H, Y0[j, :H].copy())
+        err, worst = oracle.scaled_error(Y[j, :H].copy(), y64, bound)
+        assert err <= TOL, f"rhs {j}: scaled error {err:.3e} at row {worst}"
+        if dyadic:
+            np.testing.assert_array_equal(Y[j, :H], y64.astype(np.float32))
+        # rows past the matrix (leading-dimension padding) are untouched
+        np.testing.assert_array_equal(Y[j, H:], Y0[j, H:])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("dtype", [cs.DT_BF16, cs.DT_F16])
+@pytest.mark.parametrize("k", [1, 3, 4, 8, 16, 32])
+def test_spmm_matches_oracle(ctx, oracle, name, dtype, k):
+    _run(ctx, oracle, CASES[name], dtype, k)
+
+
+def test_spmm_leading_dimensions(ctx, oracle):
+    _run(ctx, oracle, CASES["rand"], cs.DT_BF16, 8, pad=12)
+
+
+def test_spmm_argument_checks(ctx):
+    A = CASES["rand"]
+    M = cs.BlockSpmm(ctx, cs.coo2bcsr(A.copy(), 4, 4), cs.DT_BF16)
+    ldx, ldy = (A.width + 3) // 4 * 4, (A.height + 3) // 4 * 4
+    dX = cs.DeviceBuffer(ctx, np.zeros((4, ldx), np.uint16))
+    dY = cs.DeviceBuffer(ctx, np.zeros((4, ldy), np.float32))
+    for bad in [(4, ldx + 2, ldy), (4, ldx, ldy - 4), (4, A.width - 4, ldy)]:
+        with pytest.raises(cs.ClspmvError):
+            M.spmm(bad[0], dX, bad[1], dY, bad[2])
+    with pytest.raises(cs.ClspmvError):  # 2x4 blocks are not MFMA-shaped
+        cs.BlockSpmm(ctx, cs.coo2bcsr(A.copy(), 4, 2), cs.DT_BF16)
+    with pytest.raises(cs.ClspmvError):  # SpMV entry refuses the SpMM-only format
+        x = cs.DeviceVector(ctx, A.width, np.zeros(A.width, np.float32))
+        y = cs.DeviceVector(ctx, A.height)
+        check = cs.check
+        check(cs.lib.clspmv_spmv(ctx.h, M.h, 0, x.p, y.p), "spmv")
