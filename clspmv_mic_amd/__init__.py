@@ -152,7 +152,7 @@ class DmatInfo(C.Structure):
     _fields_ = [("format", C.c_int), ("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int),
                 ("stored", C.c_longlong), ("device_bytes", C.c_size_t), ("bheight", C.c_int),
                 ("bwidth", C.c_int), ("slice_height", C.c_int), ("panel_width", C.c_int),
-                ("panel_active", C.c_int), ("plain", C.c_int)]
+                ("panel_active", C.c_int), ("plain", C.c_int), ("dtype", C.c_int)]
 
 
 class DriverReport(C.Structure):
@@ -233,13 +233,26 @@ for _n, _T in [("csr", CsrMatrix), ("coo", CooMatrix), ("ell", EllMatrix), ("sel
     _sig(f"clspmv_upload_{_n}", C.c_int, VP, P(_T), P(VP))
 _sig("clspmv_dmat_free", C.c_int, VP)
 _sig("clspmv_dmat_get_info", C.c_int, VP, P(DmatInfo))
-_sig("clspmv_upload_coo_ex", C.c_int, VP, P(CooMatrix), C.c_int, P(VP))
-_sig("clspmv_upload_csr_ex", C.c_int, VP, P(CsrMatrix), C.c_int, P(VP))
+class UploadOpts(C.Structure):
+    _fields_ = [("panel_width", C.c_int), ("dtype", C.c_int)]
+
+
+_sig("clspmv_upload_coo_ex", C.c_int, VP, P(CooMatrix), P(UploadOpts), P(VP))
+_sig("clspmv_upload_csr_ex", C.c_int, VP, P(CsrMatrix), P(UploadOpts), P(VP))
+_sig("clspmv_upload_dia_ex", C.c_int, VP, P(DiaMatrix), P(UploadOpts), P(VP))
 _sig("clspmv_dmat_set_panel", C.c_int, VP, C.c_int)
 _sig("clspmv_upload_bcsr_lowp", C.c_int, VP, P(BcsrMatrix), C.c_int, P(VP))
 _sig("clspmv_spmm", C.c_int, VP, VP, C.c_int, VP, C.c_int, VP, C.c_int)
 _sig("clspmv_convert_from_f32", C.c_int, PF, VP, C.c_size_t, C.c_int)
 _sig("clspmv_convert_to_f32", C.c_int, VP, PF, C.c_size_t, C.c_int)
+_sig("clspmv_server_create", C.c_int, VP, P(VP))
+_sig("clspmv_server_destroy", C.c_int, VP)
+_sig("clspmv_server_add_matrix", C.c_int, VP, VP, C.c_int, C.c_int, P(C.c_int))
+_sig("clspmv_server_submit", C.c_int, VP, VP, P(C.c_longlong))
+_sig("clspmv_server_wait", C.c_int, VP, C.c_longlong, P(C.c_double))
+_sig("clspmv_server_clock", C.c_double, VP)
+_sig("clspmv_server_replay", C.c_int, VP, C.c_int, VP, C.c_int, P(C.c_double), P(C.c_double))
+_sig("clspmv_request_x", None, C.c_ulonglong, C.c_int, C.c_int, PF)
 _sig("clspmv_spmv", C.c_int, VP, VP, C.c_int, VP, VP)
 _sig("clspmv_time_spmv", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int, C.c_int, P(C.c_double))
 _sig("clspmv_bandwidth_probe", C.c_int, VP, C.c_size_t, C.c_int, P(C.c_double), P(C.c_double))
@@ -623,6 +636,75 @@ class DeviceBuffer:
             pass
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device_ordinal", C.c_int), ("streams", C.c_int), ("max_batch", C.c_int),
+                ("hbm_budget", C.c_size_t), ("use_graphs", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("matrix", C.c_int), ("k", C.c_int), ("seed", C.c_ulonglong), ("host_y", PF),
+                ("arrival", C.c_double)]
+
+
+def request_x(seed: int, j: int, n: int) -> np.ndarray:
+    """x_j of a served request, as the device generates it."""
+    out = np.empty(n, np.float32)
+    lib.clspmv_request_x(C.c_ulonglong(seed), j, n, _fp(out))
+    return out
+
+
+class Server:
+    """The request-serving loop (include/clspmv_serve.h)."""
+
+    def __init__(self, device: int = 0, streams: int = 4, max_batch: int = 64,
+                 hbm_budget: int = 8 << 30, use_graphs: bool = True):
+        cfg = ServerConfig(device, streams, max_batch, hbm_budget, int(use_graphs))
+        h = VP()
+        check(lib.clspmv_server_create(C.byref(cfg), C.byref(h)), "server_create")
+        self.h = h
+        self._keep = []  # matrices must outlive the server
+
+    def add(self, dmat, method: int = METHOD_AUTO, kmax: int = 16) -> int:
+        i = C.c_int()
+        check(lib.clspmv_server_add_matrix(self.h, dmat.h, method, kmax, C.byref(i)), "server_add_matrix")
+        self._keep.append(dmat)
+        return i.value
+
+    def submit(self, matrix: int, k: int, seed: int, host_y: Optional[np.ndarray] = None) -> int:
+        r = Request(matrix, k, seed, _fp(host_y) if host_y is not None else None, 0.0)
+        t = C.c_longlong()
+        check(lib.clspmv_server_submit(self.h, C.byref(r), C.byref(t)), "server_submit")
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        lat = C.c_double()
+        check(lib.clspmv_server_wait(self.h, C.c_longlong(ticket), C.byref(lat)), "server_wait")
+        return lat.value
+
+    def replay(self, reqs, mode: int = 0):
+        """reqs: list of (matrix, k, seed, arrival_s, host_y or None).
+        Returns (latencies, wall seconds)."""
+        arr = (Request * len(reqs))()
+        for i, (m, k, seed, t, y) in enumerate(reqs):
+            arr[i] = Request(m, k, seed, _fp(y) if y is not None else None, t)
+        lat = np.zeros(len(reqs), np.float64)
+        wall = C.c_double()
+        check(lib.clspmv_server_replay(self.h, len(reqs), arr, mode, lat.ctypes.data_as(P(C.c_double)),
+                                       C.byref(wall)), "server_replay")
+        return lat, wall.value
+
+    def close(self):
+        if self.h:
+            lib.clspmv_server_destroy(self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class BlockSpmm:
     """A 4x4-block BCSR matrix with BF16 / FP16 values on the device and its
     MFMA SpMM: Y += A X (clspmv_upload_bcsr_lowp / clspmv_spmm)."""
@@ -647,18 +729,23 @@ class BlockSpmm:
 
 class DeviceMatrix:
     """A device-resident matrix of any storage format.  For COO and CSR,
-    ``panel`` selects the device encoding (clspmv_upload_{coo,csr}_ex):
-    0 own arrays, a width, PANEL_AUTO or PANEL_BOTH; None is the plain
-    upload call."""
+    ``panel`` selects the device encoding (0 own arrays, a width, PANEL_AUTO
+    or PANEL_BOTH); for COO, CSR and DIA, ``dtype`` the stored value type
+    (DT_F32, DT_BF16, DT_F16) -- clspmv_upload_{coo,csr,dia}_ex.  Both None:
+    the plain upload call."""
 
-    def __init__(self, ctx: Context, m, panel: Optional[int] = None):
+    def __init__(self, ctx: Context, m, panel: Optional[int] = None, dtype: Optional[int] = None):
         s = m.s if isinstance(m, Owned) else (m.c if isinstance(m, Coo) else m)
         h = VP()
         name = _UPLOAD[type(s)]
-        if panel is not None:
-            if name not in ("coo", "csr"):
-                raise ValueError("panel encodings exist for COO and CSR only")
-            check(getattr(lib, f"clspmv_upload_{name}_ex")(ctx.h, C.byref(s), int(panel), C.byref(h)),
+        if panel is not None or dtype is not None:
+            if name not in ("coo", "csr", "dia"):
+                raise ValueError("encoding options exist for COO, CSR and DIA only")
+            default_pw = PANEL_AUTO if name == "coo" else 0
+            if name == "csr" and dtype not in (None, DT_F32) and panel is None:
+                default_pw = PANEL_AUTO
+            opts = UploadOpts(default_pw if panel is None else int(panel), DT_F32 if dtype is None else dtype)
+            check(getattr(lib, f"clspmv_upload_{name}_ex")(ctx.h, C.byref(s), C.byref(opts), C.byref(h)),
                   "upload")
         else:
             check(getattr(lib, f"clspmv_upload_{name}")(ctx.h, C.byref(s), C.byref(h)), "upload")

@@ -185,11 +185,13 @@ int clspmv_init_mat_kernels_on(int device, const clspmv_cocktail* m, const float
                        [&](clspmv_ctx* x, clspmv_dmat** A) { return clspmv_upload_ell(x, &m->ell, A); })) ||
         (st = add_part(g, c, m->ifusecsr, m->csr_meth_num,
                        [&](clspmv_ctx* x, clspmv_dmat** A) {
-                           return clspmv_upload_csr_ex(x, &m->csr, CLSPMV_PANEL_BOTH, A);
+                           const clspmv_upload_opts o{CLSPMV_PANEL_BOTH, CLSPMV_DT_F32};
+                           return clspmv_upload_csr_ex(x, &m->csr, &o, A);
                        })) ||
         (st = add_part(g, c, m->ifusecoo, m->coo_meth_num,
                        [&](clspmv_ctx* x, clspmv_dmat** A) {
-                           return clspmv_upload_coo_ex(x, &m->coo, CLSPMV_PANEL_BOTH, A);
+                           const clspmv_upload_opts o{CLSPMV_PANEL_BOTH, CLSPMV_DT_F32};
+                           return clspmv_upload_coo_ex(x, &m->coo, &o, A);
                        })))
         return cleanup(st);
     if ((st = capture(k))) return cleanup(st);

@@ -31,6 +31,46 @@ __device__ __forceinline__ int4 ld_stream(const int4* p) {
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
+// Stored matrix values: FP32, or BF16 / FP16 bits widened to FP32 when
+// loaded (products and sums stay FP32).  ld: one value; ld2 / ld4: two /
+// four consecutive values (8 / 16-byte aligned for FP32, 4 / 8 for 16-bit).
+struct ValF32 {
+    typedef float T;
+    static __device__ __forceinline__ float ld(const float* p) { return ld_stream(p); }
+    static __device__ __forceinline__ float2 ld2(const float* p) {
+        return *reinterpret_cast<const float2*>(p);
+    }
+    static __device__ __forceinline__ float4 ld4(const float* p) {
+        return ld_stream(reinterpret_cast<const float4*>(p));
+    }
+};
+
+template <class Cvt>
+struct Val16 {
+    typedef unsigned short T;
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    static __device__ __forceinline__ float ld(const T* p) { return Cvt::f(ld_stream(p)); }
+    static __device__ __forceinline__ float2 ld2(const T* p) {
+        const unsigned w = ld_stream(reinterpret_cast<const unsigned*>(p));
+        return make_float2(Cvt::f((T)(w & 0xFFFFu)), Cvt::f((T)(w >> 16)));
+    }
+    static __device__ __forceinline__ float4 ld4(const T* p) {
+        const u32x2 w = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+        return make_float4(Cvt::f((T)(w.x & 0xFFFFu)), Cvt::f((T)(w.x >> 16)),
+                           Cvt::f((T)(w.y & 0xFFFFu)), Cvt::f((T)(w.y >> 16)));
+    }
+};
+struct CvtBF16 {
+    static __device__ __forceinline__ float f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+};
+struct CvtF16 {
+    static __device__ __forceinline__ float f(unsigned short h) {
+        return (float)__builtin_bit_cast(_Float16, h);
+    }
+};
+typedef Val16<CvtBF16> ValBF16;
+typedef Val16<CvtF16> ValF16;
+
 // Work-groups are dispatched round-robin over the XCDs (group b runs on XCD
 // b % 8).  Remap so that XCD k processes one contiguous range of tiles: rows
 // that share x windows (banded matrices) then share one L2.  Bijective on

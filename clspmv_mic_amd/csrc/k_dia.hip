@@ -33,10 +33,10 @@ __device__ __forceinline__ float x_at(const float* __restrict__ x, int c, int co
 
 // One row per lane; U diagonals per iteration with independent
 // accumulators so that U data streams and U x reads are in flight per lane.
-template <int U>
+template <int U, class V>
 __global__ __launch_bounds__(kBlock) void dia1_kernel(
     int rows, int cols, int len, int ndiag, const int* __restrict__ off,
-    const float* __restrict__ data, const float* __restrict__ x,
+    const typename V::T* __restrict__ data, const float* __restrict__ x,
     float* __restrict__ y) {
     const int r = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (r >= rows) return;
@@ -47,13 +47,13 @@ __global__ __launch_bounds__(kBlock) void dia1_kernel(
     for (; j + U <= ndiag; j += U) {
         float d[U], xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) d[u] = ld_stream(data + (size_t)(j + u) * len + r);
+        for (int u = 0; u < U; ++u) d[u] = V::ld(data + (size_t)(j + u) * len + r);
 #pragma unroll
         for (int u = 0; u < U; ++u) xv[u] = x_at(x, r + off[j + u], cols);
 #pragma unroll
         for (int u = 0; u < U; ++u) a[u] = fmaf(d[u], xv[u], a[u]);
     }
-    for (; j < ndiag; ++j) a[0] = fmaf(ld_stream(data + (size_t)j * len + r), x_at(x, r + off[j], cols), a[0]);
+    for (; j < ndiag; ++j) a[0] = fmaf(V::ld(data + (size_t)j * len + r), x_at(x, r + off[j], cols), a[0]);
     float sum = 0.0f;
 #pragma unroll
     for (int u = 0; u < U; ++u) sum += a[u];
@@ -62,10 +62,10 @@ __global__ __launch_bounds__(kBlock) void dia1_kernel(
 
 // Two consecutive rows per lane (64-bit data loads, len % 2 == 0), U
 // diagonals per iteration.
-template <int U>
+template <int U, class V>
 __global__ __launch_bounds__(kBlock) void dia2_kernel(
     int rows, int cols, int len, int ndiag, const int* __restrict__ off,
-    const float* __restrict__ data, const float* __restrict__ x,
+    const typename V::T* __restrict__ data, const float* __restrict__ x,
     float* __restrict__ y) {
     const int r = (xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) * 2;
     if (r >= rows) return;
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void dia2_kernel(
         float x0[U], x1[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            d[u] = *reinterpret_cast<const float2*>(data + (size_t)(j + u) * len + r);
+            d[u] = V::ld2(data + (size_t)(j + u) * len + r);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int c = r + off[j + u];
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void dia2_kernel(
         }
     }
     for (; j < ndiag; ++j) {
-        const float2 d = *reinterpret_cast<const float2*>(data + (size_t)j * len + r);
+        const float2 d = V::ld2(data + (size_t)j * len + r);
         const int c = r + off[j];
         a0[0] = fmaf(d.x, x_at(x, c, cols), a0[0]);
         a1[0] = fmaf(d.y, x_at(x, c + 1, cols), a1[0]);
@@ -109,10 +109,10 @@ __global__ __launch_bounds__(kBlock) void dia2_kernel(
 
 // RPT rows per lane (strided by the work-group size) with the work-group's x
 // window in LDS; the host guarantees RPT*kBlock + span <= kLdsWindow.
-template <int RPT>
+template <int RPT, class V>
 __global__ __launch_bounds__(kBlock) void dia_lds_kernel(
     int rows, int cols, int len, int ndiag, const int* __restrict__ off,
-    int min_off, int span, const float* __restrict__ data,
+    int min_off, int span, const typename V::T* __restrict__ data,
     const float* __restrict__ x, float* __restrict__ y) {
     __shared__ float win[launch::kLdsWindow];
     const int t = threadIdx.x;
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void dia_lds_kernel(
         for (int q = 0; q < RPT; ++q) {
             const int rl = q * kBlock + t;
             if (r0 + rl < rows)
-                a[q] = fmaf(ld_stream(data + (size_t)j * len + r0 + rl), win[rl + o], a[q]);
+                a[q] = fmaf(V::ld(data + (size_t)j * len + r0 + rl), win[rl + o], a[q]);
         }
     }
 #pragma unroll
@@ -187,35 +187,54 @@ int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 namespace launch {
 
-hipError_t dia(hipStream_t s, int rows, int cols, int len, int ndiag,
-               const int* offsets, int min_off, int max_off, int method,
-               const float* data, const float* x, float* y) {
-    if (rows <= 0 || ndiag <= 0) return hipSuccess;
+template <class V>
+hipError_t dia_typed(hipStream_t s, int rows, int cols, int len, int ndiag,
+                     const int* offsets, int min_off, int max_off, int method,
+                     const typename V::T* data, const float* x, float* y) {
     const int span = max_off - min_off;
     switch (method) {
         case 0:
-            hipLaunchKernelGGL(dia1_kernel<8>, dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s,
+            hipLaunchKernelGGL((dia1_kernel<8, V>), dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s,
                                rows, cols, len, ndiag, offsets, data, x, y);
             break;
         case 1:
             if (len % 2) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(dia2_kernel<4>, dim3(ceil_div(rows, 2 * kBlock)), dim3(kBlock), 0, s,
+            hipLaunchKernelGGL((dia2_kernel<4, V>), dim3(ceil_div(rows, 2 * kBlock)), dim3(kBlock), 0, s,
                                rows, cols, len, ndiag, offsets, data, x, y);
             break;
         case 2:
             if (kBlock + span > kLdsWindow) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(dia_lds_kernel<1>, dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s,
+            hipLaunchKernelGGL((dia_lds_kernel<1, V>), dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s,
                                rows, cols, len, ndiag, offsets, min_off, span, data, x, y);
             break;
         case 3:
             if (4 * kBlock + span > kLdsWindow) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(dia_lds_kernel<4>, dim3(ceil_div(rows, 4 * kBlock)), dim3(kBlock), 0, s,
+            hipLaunchKernelGGL((dia_lds_kernel<4, V>), dim3(ceil_div(rows, 4 * kBlock)), dim3(kBlock), 0, s,
                                rows, cols, len, ndiag, offsets, min_off, span, data, x, y);
             break;
         default:
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t dia(hipStream_t s, int rows, int cols, int len, int ndiag,
+               const int* offsets, int min_off, int max_off, int method, int dtype,
+               const void* data, const float* x, float* y) {
+    if (rows <= 0 || ndiag <= 0) return hipSuccess;
+    switch (dtype) {
+        case 0:
+            return dia_typed<ValF32>(s, rows, cols, len, ndiag, offsets, min_off, max_off, method,
+                                     static_cast<const float*>(data), x, y);
+        case 1:
+            return dia_typed<ValBF16>(s, rows, cols, len, ndiag, offsets, min_off, max_off, method,
+                                      static_cast<const unsigned short*>(data), x, y);
+        case 2:
+            return dia_typed<ValF16>(s, rows, cols, len, ndiag, offsets, min_off, max_off, method,
+                                     static_cast<const unsigned short*>(data), x, y);
+        default:
+            return hipErrorInvalidValue;
+    }
 }
 
 hipError_t bdia(hipStream_t s, int rows, int cols, int len, int nband,

@@ -124,10 +124,10 @@ __device__ __forceinline__ void seg_scan_write(const SubChunk& q, const float* _
     if (lane == kWave - 1) out[before + nf - 1] = s;
 }
 
-template <int NT>
+template <int NT, class V>
 __global__ __launch_bounds__(NT) void panel_pass1_kernel(
     const int4* __restrict__ wg, int pw, int cols,
-    const unsigned short* __restrict__ pcol, const float* __restrict__ pval,
+    const unsigned short* __restrict__ pcol, const typename V::T* __restrict__ pval,
     const int* __restrict__ sc_seg, const int* __restrict__ sc_start,
     const float* __restrict__ x, bool xa, float* __restrict__ seg_val) {
     extern __shared__ float xs[];
@@ -188,7 +188,8 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
                 const int i0 = sc_start[c] + lane * 4;
                 if (i0 < sc_start[c + 1]) {
                     q[u].c = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(pcol + i0));
-                    q[u].v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(pval + i0));
+                    const float4 v = V::ld4(pval + i0);
+                    q[u].v = f32x4{v.x, v.y, v.z, v.w};
                 }
             }
         }
@@ -252,25 +253,48 @@ __global__ __launch_bounds__(launch::kBlockThreads) void panel_pass2_kernel(
 
 namespace launch {
 
+hipError_t panel_prepare() {
+    // Pass 1 holds up to kPanelMax floats of x in dynamic LDS, beyond the
+    // default limit: raise it once per device, outside any graph capture.
+    const int lds = (int)(sizeof(float) * kPanelMax);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_pass1_kernel<kPanelThreads, ValF32>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_pass1_kernel<kPanelThreads, ValBF16>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_pass1_kernel<kPanelThreads, ValF16>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    return e;
+}
+
 hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
-                      const unsigned short* pcol, const float* pval,
+                      const unsigned short* pcol, int dtype, const void* pval,
                       const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
                       float* seg_val, int npanels, int nblocks, const int* blk_row,
                       const int* blk_tab, const int* seg_row, float* y) {
     if (nwg <= 0 || nblocks <= 0) return hipSuccess;
     if (pw <= 0 || pw > kPanelMax) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * (size_t)pw;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&panel_pass1_kernel<kPanelThreads>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float) * kPanelMax));
-        if (e != hipSuccess) return e;
-        attr_set = true;
+    const int4* wg4 = reinterpret_cast<const int4*>(wg);
+    const unsigned short* p16 = static_cast<const unsigned short*>(pval);
+    switch (dtype) {
+        case 0:
+            hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValF32>), dim3(nwg), dim3(kPanelThreads), lds,
+                               s, wg4, pw, cols, pcol, static_cast<const float*>(pval), sc_seg, sc_start, x,
+                               x_aligned, seg_val);
+            break;
+        case 1:
+            hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValBF16>), dim3(nwg), dim3(kPanelThreads),
+                               lds, s, wg4, pw, cols, pcol, p16, sc_seg, sc_start, x, x_aligned, seg_val);
+            break;
+        case 2:
+            hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValF16>), dim3(nwg), dim3(kPanelThreads),
+                               lds, s, wg4, pw, cols, pcol, p16, sc_seg, sc_start, x, x_aligned, seg_val);
+            break;
+        default:
+            return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(panel_pass1_kernel<kPanelThreads>, dim3(nwg), dim3(kPanelThreads), lds, s,
-                       reinterpret_cast<const int4*>(wg), pw, cols, pcol, pval, sc_seg, sc_start,
-                       x, x_aligned, seg_val);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(panel_pass2_kernel, dim3(nblocks), dim3(kBlockThreads), 0, s, npanels,

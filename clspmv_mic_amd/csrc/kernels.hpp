@@ -52,8 +52,11 @@ constexpr int kBlockRowsMax = 4096;   // rows of a pass-2 row block
 constexpr int kBlockThreads = 1024;   // pass-2 work-group size
 constexpr int kBlockItems = 8;        // segments per pass-2 thread
 constexpr int kBlockSegsMax = kBlockThreads * kBlockItems;
+// Raises pass 1's dynamic LDS limit on the current device (call before the
+// first launch, outside graph capture).
+hipError_t panel_prepare();
 hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
-                      const unsigned short* pcol, const float* pval,
+                      const unsigned short* pcol, int dtype, const void* pval,
                       const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
                       float* seg_val, int npanels, int nblocks, const int* blk_row,
                       const int* blk_tab, const int* seg_row, float* y);
@@ -68,9 +71,10 @@ hipError_t sell(hipStream_t s, int rows, int slice_height, const int* ptr,
 // rows = number of rows computed (min(height, dia_length_aligned)).
 // Methods: 0 one row per lane, 1 two rows per lane (len % 2 == 0),
 // 2 / 3 one / four rows per lane with an LDS x window.
+// dtype: 0 FP32 values, 1 BF16, 2 FP16 (data holds 16-bit bits).
 hipError_t dia(hipStream_t s, int rows, int cols, int len, int ndiag,
-               const int* offsets, int min_off, int max_off, int method,
-               const float* data, const float* x, float* y);
+               const int* offsets, int min_off, int max_off, int method, int dtype,
+               const void* data, const float* x, float* y);
 // Largest x window (floats) a work-group stages in LDS.
 constexpr int kLdsWindow = 8192;
 // LDS-window methods 1, 2, 4 only; methods 0 and 3 are run by dia() over

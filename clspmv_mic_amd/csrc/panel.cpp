@@ -21,7 +21,7 @@ int auto_panel_width(int cols, long long nnz) {
 }
 
 int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* row,
-                const int* col, const float* val, int pw) {
+                const int* col, const float* val, int pw, int dtype) {
     constexpr int kSub = launch::kPanelSub;
     if (pw <= 0 || pw % 8 || pw > launch::kPanelMax)
         return fail(CLSPMV_ERR_INVALID, "panel width %d: need a multiple of 8 in [8, %d]", pw,
@@ -185,13 +185,24 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
         }
     }
 
+    {
+        const hipError_t e = launch::panel_prepare();
+        if (e != hipSuccess)
+            return fail(CLSPMV_ERR_HIP, "panel kernel LDS limit: %s", hipGetErrorString(e));
+    }
     d->pw = pw;
     d->pnwg = (int)(wg.size() / 4);
     d->pnp = P;
     d->pnb = nb;
     int st = upload_array(d, wg.data(), wg.size(), &d->pwg);
     if (!st) st = upload_array(d, pc.data(), pc.size(), &d->pcol);
-    if (!st) st = upload_array(d, pv.data(), pv.size(), &d->pval);
+    if (!st && dtype == CLSPMV_DT_F32) {
+        st = upload_array(d, pv.data(), pv.size(), &d->pval);
+    } else if (!st) {
+        std::vector<unsigned short> lp(pv.size());
+        st = clspmv_convert_from_f32(pv.data(), lp.data(), lp.size(), dtype);
+        if (!st) st = upload_array(d, lp.data(), lp.size(), &d->plpval);
+    }
     if (!st) st = upload_array(d, sc_seg.data(), sc_seg.size(), &d->psc_seg);
     if (!st) st = upload_array(d, sc_start.data(), sc_start.size(), &d->psc_start);
     if (!st) st = upload_array<float>(d, nullptr, (size_t)nseg, &d->pseg);
@@ -209,6 +220,7 @@ void drop_encoding(clspmv_dmat* d, bool panel) {
         release_array(d, &d->pwg);
         release_array(d, &d->pcol);
         release_array(d, &d->pval);
+        release_array(d, &d->plpval);
         release_array(d, &d->psc_seg);
         release_array(d, &d->psc_start);
         release_array(d, &d->pseg);

@@ -108,7 +108,8 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
     const bool xa = aligned16(x);
     if ((I.format == CLSPMV_FMT_CSR || I.format == CLSPMV_FMT_COO) && A->use_panel) {
         // The panel encoding serves every method of the format.
-        e = launch::panel_spmv(s, A->pnwg, A->pwg, A->pw, I.cols, A->pcol, A->pval, A->psc_seg,
+        e = launch::panel_spmv(s, A->pnwg, A->pwg, A->pw, I.cols, A->pcol, A->dtype,
+                               A->dtype ? (const void*)A->plpval : (const void*)A->pval, A->psc_seg,
                                A->psc_start, x, xa,
                                A->pseg, A->pnp, A->pnb, A->pblk_row, A->pblk_tab, A->pseg_row, y);
         if (e != hipSuccess)
@@ -140,7 +141,8 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
             if (method >= 2 && (method == 3 ? 4 : 1) * 256 + span > launch::kLdsWindow)
                 return fail(CLSPMV_ERR_UNSUPPORTED, "DIA method %d: offset span %d too wide for LDS", method, span);
             e = launch::dia(s, std::min(I.rows, A->len), I.cols, A->len, A->ndiag, A->offs,
-                            A->min_off, A->max_off, method, A->val, x, y);
+                            A->min_off, A->max_off, method, A->dtype,
+                            A->dtype ? (const void*)A->lpval : (const void*)A->val, x, y);
             break;
         }
         case CLSPMV_FMT_BDIA: {
@@ -152,7 +154,7 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
                 if (method == 3 && A->len % 2)
                     return fail(CLSPMV_ERR_UNSUPPORTED, "BDIA method 3 needs an even aligned length");
                 e = launch::dia(s, std::min(I.rows, A->len), I.cols, A->len, A->ndiag, A->offs_exp,
-                                A->min_off, A->max_off, method == 0 ? 0 : 1, A->val, x, y);
+                                A->min_off, A->max_off, method == 0 ? 0 : 1, 0, A->val, x, y);
                 break;
             }
             e = launch::bdia(s, std::min(I.rows, A->len), I.cols, A->len, A->nband, A->offs,
@@ -318,43 +320,54 @@ int clspmv_method_count(int format) {
         }                             \
     } while (0)
 
-// Builds the panel encoding asked for by `panel_width` (a width, or
+// Builds the panel encoding asked for by opts (a width, or
 // CLSPMV_PANEL_AUTO / CLSPMV_PANEL_BOTH) from row-sorted entries; with AUTO
 // it replaces the plain arrays, with BOTH it is added beside them.
-static int add_panel(clspmv_dmat* d, int panel_width, const int* row, const int* col,
+static int add_panel(clspmv_dmat* d, const clspmv_upload_opts& o, const int* row, const int* col,
                      const float* val) {
     const clspmv_dmat_info& I = d->info;
-    int pw = panel_width;
-    if (pw == CLSPMV_PANEL_AUTO || pw == CLSPMV_PANEL_BOTH) pw = auto_panel_width(I.cols, I.nnz);
+    int pw = o.panel_width;
+    if (pw == CLSPMV_PANEL_AUTO || pw == CLSPMV_PANEL_BOTH) {
+        pw = auto_panel_width(I.cols, I.nnz);
+        // 16-bit values exist only in the panel encoding.
+        if (pw == 0 && o.dtype != CLSPMV_DT_F32) pw = std::min(launch::kPanelMax, round_up(I.cols, 8));
+    }
     if (pw == 0) return CLSPMV_OK;
-    int st = build_panel(d, I.rows, I.cols, I.nnz, row, col, val, pw);
-    if (st == CLSPMV_ERR_UNSUPPORTED && panel_width < 0) return CLSPMV_OK;  // stay plain
+    int st = build_panel(d, I.rows, I.cols, I.nnz, row, col, val, pw, o.dtype);
+    if (st == CLSPMV_ERR_UNSUPPORTED && o.panel_width < 0 && o.dtype == CLSPMV_DT_F32)
+        return CLSPMV_OK;  // stay plain
     if (st) return st;
-    if (panel_width != CLSPMV_PANEL_BOTH) drop_encoding(d, false);
+    d->dtype = o.dtype;
+    if (o.panel_width != CLSPMV_PANEL_BOTH) drop_encoding(d, false);
     return CLSPMV_OK;
 }
 
-static int check_panel_arg(int panel_width) {
-    if (panel_width == 0 || panel_width == CLSPMV_PANEL_AUTO || panel_width == CLSPMV_PANEL_BOTH)
-        return CLSPMV_OK;
-    if (panel_width < 8 || panel_width % 8 || panel_width > launch::kPanelMax)
-        return fail(CLSPMV_ERR_INVALID, "panel width %d: need 0, -1, -2 or a multiple of 8 in [8, %d]",
-                    panel_width, launch::kPanelMax);
+static int check_opts(const clspmv_upload_opts& o) {
+    if (o.dtype != CLSPMV_DT_F32 && o.dtype != CLSPMV_DT_BF16 && o.dtype != CLSPMV_DT_F16)
+        return fail(CLSPMV_ERR_INVALID, "dtype %d", o.dtype);
+    const int pw = o.panel_width;
+    if (o.dtype != CLSPMV_DT_F32 && (pw == 0 || pw == CLSPMV_PANEL_BOTH))
+        return fail(CLSPMV_ERR_UNSUPPORTED, "16-bit COO / CSR values need the panel encoding only");
+    if (pw == 0 || pw == CLSPMV_PANEL_AUTO || pw == CLSPMV_PANEL_BOTH) return CLSPMV_OK;
+    if (pw < 8 || pw % 8 || pw > launch::kPanelMax)
+        return fail(CLSPMV_ERR_INVALID, "panel width %d: need 0, -1, -2 or a multiple of 8 in [8, %d]", pw,
+                    launch::kPanelMax);
     return CLSPMV_OK;
 }
 
 int clspmv_upload_csr(clspmv_ctx* ctx, const clspmv_csr_matrix* m, clspmv_dmat** out) {
-    return clspmv_upload_csr_ex(ctx, m, 0, out);
+    return clspmv_upload_csr_ex(ctx, m, nullptr, out);
 }
 
 int clspmv_upload_coo(clspmv_ctx* ctx, const clspmv_coo_matrix* m, clspmv_dmat** out) {
-    return clspmv_upload_coo_ex(ctx, m, CLSPMV_PANEL_AUTO, out);
+    return clspmv_upload_coo_ex(ctx, m, nullptr, out);
 }
 
-int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m, int panel_width,
-                         clspmv_dmat** out) {
+int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m,
+                         const clspmv_upload_opts* opts, clspmv_dmat** out) {
+    const clspmv_upload_opts o = opts ? *opts : clspmv_upload_opts{0, CLSPMV_DT_F32};
     {
-        int st0 = check_panel_arg(panel_width);
+        int st0 = check_opts(o);
         if (st0) return st0;
     }
     UPLOAD_PROLOGUE(m, CLSPMV_FMT_CSR)
@@ -380,20 +393,21 @@ int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m, int panel_
     std::vector<int> blocks = adaptive_blocks(m->csr_row_ptr, rows);
     d->nblocks = (int)blocks.size() - 1;
     UPLOAD_CHECK(upload_array(d, blocks.data(), blocks.size(), &d->blocks));
-    if (panel_width != 0 && nnz > 0) {
+    if (o.panel_width != 0 && nnz > 0) {
         std::vector<int> row_of(nnz);
         for (int r = 0; r < rows; ++r)
             std::fill(row_of.begin() + m->csr_row_ptr[r], row_of.begin() + m->csr_row_ptr[r + 1], r);
-        UPLOAD_CHECK(add_panel(d, panel_width, row_of.data(), m->csr_col_id, m->csr_data));
+        UPLOAD_CHECK(add_panel(d, o, row_of.data(), m->csr_col_id, m->csr_data));
     }
     *out = d;
     return CLSPMV_OK;
 }
 
-int clspmv_upload_coo_ex(clspmv_ctx* ctx, const clspmv_coo_matrix* m, int panel_width,
-                         clspmv_dmat** out) {
+int clspmv_upload_coo_ex(clspmv_ctx* ctx, const clspmv_coo_matrix* m,
+                         const clspmv_upload_opts* opts, clspmv_dmat** out) {
+    const clspmv_upload_opts o = opts ? *opts : clspmv_upload_opts{CLSPMV_PANEL_AUTO, CLSPMV_DT_F32};
     {
-        int st0 = check_panel_arg(panel_width);
+        int st0 = check_opts(o);
         if (st0) return st0;
     }
     UPLOAD_PROLOGUE(m, CLSPMV_FMT_COO)
@@ -419,8 +433,8 @@ int clspmv_upload_coo_ex(clspmv_ctx* ctx, const clspmv_coo_matrix* m, int panel_
     const int chunks = (nnz + launch::kCooChunk - 1) / launch::kCooChunk;
     UPLOAD_CHECK(upload_array<int>(d, nullptr, 2 * (size_t)chunks, &d->carry_row));
     UPLOAD_CHECK(upload_array<float>(d, nullptr, 2 * (size_t)chunks, &d->carry_val));
-    if (panel_width != 0 && nnz > 0)
-        UPLOAD_CHECK(add_panel(d, panel_width, sorted.coo_row_id, sorted.coo_col_id, sorted.coo_data));
+    if (o.panel_width != 0 && nnz > 0)
+        UPLOAD_CHECK(add_panel(d, o, sorted.coo_row_id, sorted.coo_col_id, sorted.coo_data));
     *out = d;
     return CLSPMV_OK;
 }
@@ -461,6 +475,14 @@ int clspmv_upload_sell(clspmv_ctx* ctx, const clspmv_sell_matrix* m, clspmv_dmat
 }
 
 int clspmv_upload_dia(clspmv_ctx* ctx, const clspmv_dia_matrix* m, clspmv_dmat** out) {
+    return clspmv_upload_dia_ex(ctx, m, nullptr, out);
+}
+
+int clspmv_upload_dia_ex(clspmv_ctx* ctx, const clspmv_dia_matrix* m, const clspmv_upload_opts* opts,
+                         clspmv_dmat** out) {
+    const int dtype = opts ? opts->dtype : CLSPMV_DT_F32;
+    if (dtype != CLSPMV_DT_F32 && dtype != CLSPMV_DT_BF16 && dtype != CLSPMV_DT_F16)
+        return fail(CLSPMV_ERR_INVALID, "dtype %d", dtype);
     UPLOAD_PROLOGUE(m, CLSPMV_FMT_DIA)
     if (m->dia_num < 0 || m->dia_length_aligned < 0 || (m->dia_num > 0 && !m->dia_offsets))
         UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "malformed DIA matrix"));
@@ -473,7 +495,15 @@ int clspmv_upload_dia(clspmv_ctx* ctx, const clspmv_dia_matrix* m, clspmv_dmat**
     }
     d->info.stored = (long long)m->dia_num * m->dia_length_aligned;
     UPLOAD_CHECK(upload_array(d, m->dia_offsets, (size_t)m->dia_num, &d->offs));
-    UPLOAD_CHECK(upload_array(d, m->dia_data, (size_t)m->dia_num * m->dia_length_aligned, &d->val));
+    const size_t total = (size_t)m->dia_num * m->dia_length_aligned;
+    if (dtype == CLSPMV_DT_F32) {
+        UPLOAD_CHECK(upload_array(d, m->dia_data, total, &d->val));
+    } else {
+        std::vector<unsigned short> lp(total);
+        UPLOAD_CHECK(clspmv_convert_from_f32(m->dia_data, lp.data(), total, dtype));
+        UPLOAD_CHECK(upload_array(d, lp.data(), total, &d->lpval));
+        d->dtype = dtype;
+    }
     *out = d;
     return CLSPMV_OK;
 }
@@ -737,6 +767,7 @@ int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out) {
     out->panel_width = m->pw;
     out->panel_active = m->use_panel ? 1 : 0;
     out->plain = m->plain ? 1 : 0;
+    out->dtype = m->dtype;
     return CLSPMV_OK;
 }
 

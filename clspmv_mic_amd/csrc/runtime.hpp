@@ -48,6 +48,7 @@ struct clspmv_dmat {
     int* pwg = nullptr;  // int4 per pass-1 work-group
     unsigned short* pcol = nullptr;
     float* pval = nullptr;
+    unsigned short* plpval = nullptr;  // values in BF16 / FP16 (dtype)
     int* psc_seg = nullptr;
     int* psc_start = nullptr;  // first entry of each sub-chunk (+ end)
     float* pseg = nullptr;
@@ -56,8 +57,10 @@ struct clspmv_dmat {
     int* pblk_tab = nullptr;   // first segment of (block, panel)
     bool use_panel = false;
     bool plain = true;
-    // BSPMM: 16 BF16 / FP16 values per 4x4 block, row-major; rp / col hold
-    // the block rows and element columns.
+    // Values stored in 16 bits (dtype CLSPMV_DT_BF16 / CLSPMV_DT_F16): DIA
+    // data, and BSPMM blocks (16 values per 4x4 block, row-major; rp / col
+    // hold the block rows and element columns).  The panel encoding keeps
+    // its own 16-bit values in plpval.
     unsigned short* lpval = nullptr;
     int dtype = 0;
 };
@@ -110,7 +113,7 @@ void release_array(clspmv_dmat* m, T** p) {
 // and uploaded into `d`.  pw: panel width (multiple of 8, at most
 // launch::kPanelMax).
 int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* row,
-                const int* col, const float* val, int pw);
+                const int* col, const float* val, int pw, int dtype);
 // Panel width the encoding rule picks for a matrix, or 0 when panels do not
 // pay (too few nonzeros per panel to amortize staging x in LDS).
 int auto_panel_width(int cols, long long nnz);

@@ -86,6 +86,7 @@ typedef struct {
     int    panel_width;     /* COO / CSR x-panel encoding width, or 0   */
     int    panel_active;    /* 1 when launches use the panel encoding   */
     int    plain;           /* 1 when the format's own arrays are held  */
+    int    dtype;           /* enum clspmv_dtype of the stored values    */
 } clspmv_dmat_info;
 
 /* Upload a host matrix (the reference's cpy_<fmt>_mat,
@@ -101,29 +102,47 @@ int clspmv_upload_b4ell(clspmv_ctx* ctx, const clspmv_b4ell_matrix* m, clspmv_dm
 int clspmv_upload_sbell(clspmv_ctx* ctx, const clspmv_sbell_matrix* m, clspmv_dmat** out);
 int clspmv_dmat_free(clspmv_dmat* m);
 
-/* COO and CSR uploads with a choice of device encoding.  Besides the
- * format's own arrays the engine has an x-panel encoding for matrices whose
- * column indices have no locality: columns are cut into panels of
- * panel_width floats (a multiple of 8, at most 32760) so that a work-group
- * holds its panel of x in LDS, and entries are stored as 16-bit in-panel
- * columns with segment marks (6 bytes per nonzero).  Results are the same
- * to rounding; the summation order is fixed by the encoding.
- *   panel_width 0                  the format's own arrays only
- *   panel_width > 0                the panel encoding only, that width
- *   CLSPMV_PANEL_AUTO (-1)         panels when the matrix has enough
- *                                  nonzeros per panel, else the own arrays
- *   CLSPMV_PANEL_BOTH (-2)         both encodings where panels pay (panel
- *                                  active); clspmv_dmat_set_panel switches
- * clspmv_upload_coo is clspmv_upload_coo_ex(..., CLSPMV_PANEL_AUTO, ...),
- * clspmv_upload_csr is clspmv_upload_csr_ex(..., 0, ...).  With the panel
- * encoding active, every method number of the format runs the panel
- * kernels. */
+/* Uploads with a choice of device encoding and value type (no counterpart
+ * in the reference).
+ *
+ * panel_width (COO and CSR): besides the format's own arrays the engine has
+ * an x-panel encoding for matrices whose column indices have no locality:
+ * columns are cut into panels of panel_width floats (a multiple of 8, at
+ * most 32760) so that a work-group holds its panel of x in LDS, and entries
+ * are stored as 16-bit in-panel columns with segment marks (6 bytes per
+ * nonzero with FP32 values, 4 with 16-bit values).  Results are the same to
+ * rounding; the summation order is fixed by the encoding.
+ *   0                        the format's own arrays only
+ *   > 0                      the panel encoding only, that width
+ *   CLSPMV_PANEL_AUTO (-1)   panels when the matrix has enough nonzeros
+ *                            per panel, else the own arrays
+ *   CLSPMV_PANEL_BOTH (-2)   both encodings where panels pay (panel
+ *                            active); clspmv_dmat_set_panel switches
+ * With the panel encoding active, every method number of the format runs
+ * the panel kernels.
+ *
+ * dtype: CLSPMV_DT_F32 (the reference's values), or CLSPMV_DT_BF16 /
+ * CLSPMV_DT_F16: values rounded to 16 bits (nearest even) and widened to
+ * FP32 in the kernels, which accumulate in FP32 as before.  16-bit values
+ * exist for DIA (every method) and for the COO / CSR panel encoding (AUTO
+ * then always uses panels; 0 and BOTH are refused).
+ *
+ * clspmv_upload_coo is clspmv_upload_coo_ex with {CLSPMV_PANEL_AUTO, F32},
+ * clspmv_upload_csr is clspmv_upload_csr_ex with {0, F32}, clspmv_upload_dia
+ * is clspmv_upload_dia_ex with F32; opts = NULL means those defaults. */
+enum clspmv_dtype { CLSPMV_DT_F32 = 0, CLSPMV_DT_BF16 = 1, CLSPMV_DT_F16 = 2 };
 #define CLSPMV_PANEL_AUTO (-1)
 #define CLSPMV_PANEL_BOTH (-2)
-int clspmv_upload_coo_ex(clspmv_ctx* ctx, const clspmv_coo_matrix* m, int panel_width,
-                         clspmv_dmat** out);
-int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m, int panel_width,
-                         clspmv_dmat** out);
+typedef struct {
+    int panel_width;
+    int dtype;
+} clspmv_upload_opts;
+int clspmv_upload_coo_ex(clspmv_ctx* ctx, const clspmv_coo_matrix* m,
+                         const clspmv_upload_opts* opts, clspmv_dmat** out);
+int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m,
+                         const clspmv_upload_opts* opts, clspmv_dmat** out);
+int clspmv_upload_dia_ex(clspmv_ctx* ctx, const clspmv_dia_matrix* m,
+                         const clspmv_upload_opts* opts, clspmv_dmat** out);
 /* Selects the panel encoding (1) or the format's own arrays (0) of a matrix
  * that holds both. */
 int clspmv_dmat_set_panel(clspmv_dmat* m, int use_panel);
@@ -159,7 +178,6 @@ int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
  * clspmv_convert_from_f32 / clspmv_convert_to_f32: host conversions of n
  *   values between float and dtype (round to nearest even), for building X
  *   and for exact references. */
-enum clspmv_dtype { CLSPMV_DT_F32 = 0, CLSPMV_DT_BF16 = 1, CLSPMV_DT_F16 = 2 };
 int clspmv_upload_bcsr_lowp(clspmv_ctx* ctx, const clspmv_bcsr_matrix* m, int dtype,
                             clspmv_dmat** out);
 int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, int ldx,

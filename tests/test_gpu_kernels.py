@@ -45,7 +45,7 @@ def _is_dyadic(a: np.ndarray) -> bool:
     return bool(np.all(a * 64 == np.round(a * 64)))
 
 
-def _run(ctx, oracle, A: cs.Coo, owned, method, panel=None, x_offset=0):
+def _run(ctx, oracle, A: cs.Coo, owned, method, panel=None, x_offset=0, dtype=None):
     dyadic = _is_dyadic(A.val)
     if dyadic:
         x = matrices.dyadic_vector(A.width, seed=11)
@@ -53,9 +53,15 @@ def _run(ctx, oracle, A: cs.Coo, owned, method, panel=None, x_offset=0):
     else:
         x = matrices.uniform_vector(A.width, seed=11)
         y0 = matrices.uniform_vector(A.height, seed=12)
-    dA = cs.DeviceMatrix(ctx, owned, panel=panel)
+    dA = cs.DeviceMatrix(ctx, owned, panel=panel, dtype=dtype)
     if panel is not None and panel > 0:
         assert dA.info.panel_width == panel and dA.info.panel_active == 1 and dA.info.plain == 0
+    vals = A.val
+    if dtype not in (None, cs.DT_F32):
+        # The reference for 16-bit storage: the FP64 product of the values
+        # as stored.
+        assert dA.info.dtype == dtype
+        vals = cs.convert_to_f32(cs.convert_from_f32(A.val, dtype), dtype)
     if x_offset:
         # x starting x_offset floats into its buffer: unaligned for 128-bit loads
         dx = cs.DeviceVector(ctx, A.width + x_offset, np.concatenate([np.zeros(x_offset, np.float32), x]))
@@ -66,7 +72,7 @@ def _run(ctx, oracle, A: cs.Coo, owned, method, panel=None, x_offset=0):
     dA.spmv(dx, dy, method)
     ctx.sync()
     y = dy.download()
-    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+    y64, bound = oracle.spmv_f64(A.row, A.col, vals, x, A.height, y0)
     err, worst = oracle.scaled_error(y, y64, bound)
     assert err <= TOL, f"scaled error {err:.3e} at row {worst}: got {y[worst]} want {y64[worst]}"
     if dyadic:
@@ -255,3 +261,35 @@ def test_panel_bad_width_rejected(ctx):
     for bad in (4, 12, 32768, -3):
         with pytest.raises(cs.ClspmvError):
             cs.DeviceMatrix(ctx, A, panel=bad)
+
+
+@pytest.mark.parametrize("name", DIA_CASES)
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+@pytest.mark.parametrize("dtype", [cs.DT_BF16, cs.DT_F16])
+def test_dia_16bit_values(ctx, oracle, name, method, dtype):
+    """DIA with values stored in BF16 / FP16, accumulated in FP32."""
+    A = CASES[name]
+    d = cs.coo2dia(A)
+    if d is None:
+        pytest.skip("DIA refused: padded size >= 2*nnz (include/matrix_storage.h:972)")
+    offs = d.arrays()["off"]
+    if method >= 2 and (1024 if method == 3 else 256) + int(offs.max() - offs.min()) > 8192:
+        pytest.skip("offset span too wide for the LDS window")
+    _run(ctx, oracle, A, d, method, dtype=dtype)
+
+
+@pytest.mark.parametrize("name", ["rand_small", "rand_dyadic", "rand_long_dyadic", "lap2d_256", "wide"])
+@pytest.mark.parametrize("dtype", [cs.DT_BF16, cs.DT_F16])
+@pytest.mark.parametrize("fmt", ["coo", "csr"])
+def test_panel_16bit_values(ctx, oracle, name, dtype, fmt):
+    A = CASES[name]
+    owned = A.sorted() if fmt == "coo" else cs.coo2csr(A.copy())
+    _run(ctx, oracle, A, owned, 0, panel=64, dtype=dtype)
+    _run(ctx, oracle, A, owned, 0, dtype=dtype)  # automatic panel width
+
+
+def test_16bit_needs_panel(ctx):
+    A = CASES["rand_small"]
+    for panel in (0, cs.PANEL_BOTH):
+        with pytest.raises(cs.ClspmvError):
+            cs.DeviceMatrix(ctx, A.sorted(), panel=panel, dtype=cs.DT_BF16)

@@ -1,0 +1,110 @@
+"""GPU tests of the serving loop (clspmv_serve.h): every request's result is
+the same alone (plain per-request launches, one at a time) and in the mix
+(continuous batching over several streams, graphs replayed), bit for bit,
+and matches the FP64 oracle within 1e-5 of |A||x|."""
+
+import numpy as np
+import pytest
+
+import clspmv_mic_amd as cs
+from clspmv_mic_amd import matrices
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def resident(ctx):
+    """(coo, device matrix, method, dtype) of four matrices of different formats."""
+    out = []
+    lap = matrices.laplacian_2d(64)
+    out.append((lap, cs.DeviceMatrix(ctx, cs.coo2dia(lap.copy())), 0, None))
+    pl = matrices.power_law(20000, seed=2, max_len=400)
+    out.append((pl, cs.DeviceMatrix(ctx, pl.sorted(), panel=cs.PANEL_AUTO), 0, None))
+    rnd = matrices.random_sparse(3000, 2500, 0.004, seed=11, long_rows=3, long_len=900)
+    out.append((rnd, cs.DeviceMatrix(ctx, cs.coo2csr(rnd.copy())), 1, None))
+    blk = matrices.block_stencil(500, 4, 4, seed=4, offsets=(0, 1, -1, 9))
+    out.append((blk, cs.BlockSpmm(ctx, cs.coo2bcsr(blk.copy(), 4, 4), cs.DT_BF16), 0, cs.DT_BF16))
+    assert out[1][1].info.panel_active == 1
+    return out
+
+
+def _server(resident, **kw):
+    s = cs.Server(**kw)
+    ids = [s.add(d, m, kmax=8) for (_, d, m, _) in resident]
+    return s, ids
+
+
+def _requests(resident, ids, n, seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        m = int(rng.integers(len(ids)))
+        k = int(rng.integers(1, 5))
+        A = resident[m][0]
+        reqs.append((ids[m], k, 1000 + i, 0.0, np.zeros(A.height * k, np.float32)))
+    return reqs
+
+
+def _oracle_check(oracle, resident, m, k, seed, y):
+    A, _, _, dtype = resident[m]
+    vals = A.val if dtype is None else cs.convert_to_f32(cs.convert_from_f32(A.val, dtype), dtype)
+    for j in range(k):
+        x = cs.request_x(seed, j, A.width)
+        if dtype is not None:
+            x = cs.convert_to_f32(cs.convert_from_f32(x, dtype), dtype)
+        y64, bound = oracle.spmv_f64(A.row, A.col, vals, x, A.height)
+        err, worst = oracle.scaled_error(y[j * A.height:(j + 1) * A.height].copy(), y64, bound)
+        assert err <= TOL, f"matrix {m} rhs {j}: {err:.3e} at row {worst}"
+
+
+def test_alone_equals_mix_and_oracle(ctx, oracle, resident):
+    s, ids = _server(resident, streams=3, max_batch=8)
+    alone = _requests(resident, ids, 48, seed=1)
+    mixed = [(m, k, sd, t, np.zeros_like(y)) for (m, k, sd, t, y) in alone]
+    lat1, _ = s.replay(alone, mode=1)
+    lat0, _ = s.replay(mixed, mode=0)
+    s.close()
+    assert np.all(lat0 >= 0) and np.all(lat1 >= 0)
+    for (m, k, sd, _, ya), (_, _, _, _, ym) in zip(alone, mixed):
+        np.testing.assert_array_equal(ya, ym, err_msg=f"matrix {m} seed {sd}")
+    checked = set()
+    for (m, k, sd, _, y) in alone:
+        if m not in checked:
+            _oracle_check(oracle, resident, m, k, sd, y)
+            checked.add(m)
+    assert len(checked) == len(resident)
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_graphs_and_budget(ctx, resident, graphs):
+    """Plain launches and graph replays give the same results; a budget that
+    admits one request at a time still serves all of them."""
+    ref, ids = _server(resident, streams=2, max_batch=4)
+    reqs = _requests(resident, ids, 24, seed=2)
+    ref.replay(reqs, mode=1)
+    ref.close()
+    biggest = max(4 * 8 * (A.width + A.height + 8) for (A, _, _, _) in resident)
+    s, _ = _server(resident, streams=4, max_batch=16, hbm_budget=biggest, use_graphs=graphs)
+    got = [(m, k, sd, 1e-4 * i, np.zeros_like(y)) for i, (m, k, sd, _, y) in enumerate(reqs)]
+    s.replay(got, mode=0)
+    s.close()
+    for (_, _, _, _, a), (_, _, _, _, b) in zip(reqs, got):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_submit_and_wait(ctx, resident):
+    s, ids = _server(resident, streams=2, max_batch=4)
+    A = resident[0][0]
+    ys = [np.zeros(A.height * 2, np.float32) for _ in range(5)]
+    tickets = [s.submit(ids[0], 2, 77, y) for y in ys]
+    for t in tickets:
+        assert s.wait(t) >= 0
+    for y in ys[1:]:
+        np.testing.assert_array_equal(y, ys[0])
+    with pytest.raises(cs.ClspmvError):
+        s.submit(len(ids), 1, 1)        # no such matrix
+    with pytest.raises(cs.ClspmvError):
+        s.submit(ids[0], 9, 1)          # k above kmax
+    s.close()
