@@ -97,12 +97,175 @@ def cpu_baseline(A, x, y_gpu, budget_s=10.0):
     }
 
 
+def spmm_bytes(blocks, brows, rows, cols, k, s_v=2, s_x=2):
+    """Algorithmic bytes of Y += A X with 4x4 blocks: dense block values,
+    block column ids and block row pointers, X once, Y read and written."""
+    return blocks * 16 * s_v + 4 * blocks + 4 * (brows + 1) + s_x * cols * k + 2 * 4 * rows * k
+
+
+def run_spmm(args):
+    """Mixed-precision MFMA block SpMM on the 4x4-block matrix (524,288 block
+    rows x 12 blocks): one JSON line per k."""
+    import numpy as np
+    import clspmv_mic_amd as cs
+    from clspmv_mic_amd import matrices
+    import oracle
+    dtype = {"bf16": cs.DT_BF16, "fp16": cs.DT_F16}[args.dtype]
+    t0 = time.time()
+    A = matrices.block_stencil(524_288, 4, 4, seed=4)
+    bc = cs.coo2bcsr(A.copy(), 4, 4)
+    hdr = bc.header()
+    blocks, brows = hdr["bcsr_block_num"], hdr["bcsr_row_num"]
+    ctx = cs.Context(0)
+    M = cs.BlockSpmm(ctx, bc, dtype)
+    print(f"[bench] spmm matrix {A.height}x{A.width} nnz={A.nnz} blocks={blocks} ({time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    vals_s = cs.convert_to_f32(cs.convert_from_f32(A.val, dtype), dtype)
+    orc = oracle.Oracle()
+    for k in args.k:
+        ldx, ldy = A.width, A.height
+        rng = np.random.default_rng(5)
+        X = rng.uniform(-1, 1, size=(k, ldx)).astype(np.float32)
+        Xb = cs.convert_from_f32(X, dtype)
+        Xs = cs.convert_to_f32(Xb, dtype)
+        dX = cs.DeviceBuffer(ctx, Xb)
+        dY = cs.DeviceBuffer(ctx, np.zeros((k, ldy), np.float32))
+        M.spmm(k, dX, ldx, dY, ldy)
+        ctx.sync()
+        Y1 = dY.download()
+        for _ in range(args.warmup):
+            M.spmm(k, dX, ldx, dY, ldy)
+        ctx.sync()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            M.spmm(k, dX, ldx, dY, ldy)
+        ctx.sync()
+        ms = (time.perf_counter() - t1) * 1e3 / args.steps
+        flops = 2.0 * A.nnz * k
+        nbytes = spmm_bytes(blocks, brows, A.height, A.width, k)
+        # Error per right-hand side against the FP64 oracle: of the stored
+        # (rounded) values -- the kernel's own error -- and of the FP32
+        # inputs before rounding -- what 16-bit storage costs.
+        errs, qerrs = [], []
+        for j in range(min(k, args.err_columns)):
+            y64, bound = orc.spmv_f64(A.row, A.col, vals_s, Xs[j], A.height)
+            errs.append(orc.scaled_error(Y1[j], y64, bound)[0])
+            y64q, boundq = orc.spmv_f64(A.row, A.col, A.val, X[j], A.height)
+            qerrs.append(orc.scaled_error(Y1[j], y64q, boundq)[0])
+        out = {
+            "metric": "block SpMM GFLOP/s (BF16/FP16 values, FP32 accumulation on MFMA)",
+            "value": round(flops / (ms * 1e-3) / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 6),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
+            "data": "synthetic",
+            "config": {"workload": "4x4-block matrix, 524,288 block rows x 12 blocks, X uniform[-1,1]",
+                       "rows": A.height, "nnz": A.nnz, "blocks": blocks, "k": k},
+            "roofline": {"bound": "hbm", "achieved": round(nbytes / (ms * 1e-3) / 1e9, 2), "peak": PEAK_HBM_GBPS,
+                         "unit": "GB/s", "frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
+                         "traffic": None, "algorithmic_bytes": nbytes},
+            "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
+            "error_vs_fp64_stored_values": [float(f"{e:.3e}") for e in errs],
+            "error_vs_fp64_fp32_inputs": [float(f"{e:.3e}") for e in qerrs],
+            "cpu_baseline": None,
+        }
+        print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_serve(args):
+    """Serving loop over 64 resident matrices (32 stencils, 32 power-law,
+    10k..2M rows log-uniform, FP32 and BF16 alternating) under a Poisson
+    stream of requests with 1..16 right-hand sides."""
+    import numpy as np
+    import clspmv_mic_amd as cs
+    from clspmv_mic_amd import matrices
+    rng = np.random.default_rng(6)
+    nmat = args.matrices
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e6), size=nmat)).astype(np.int64)
+    ctx = cs.Context(0)
+    server = cs.Server(streams=args.streams, max_batch=64, hbm_budget=16 << 30)
+    dev, ids, kinds = [], [], []
+    t0 = time.time()
+    for i in range(nmat):
+        m = int(sizes[i])
+        bf16 = (i // 2) % 2 == 1
+        dt = cs.DT_BF16 if bf16 else cs.DT_F32
+        if i % 2 == 0:  # stencil: 2D and 3D alternating
+            if (i // 2) % 4 < 2:
+                A = matrices.laplacian_2d(max(8, int(round(m ** 0.5))))
+            else:
+                A = matrices.stencil_27pt(max(4, int(round(m ** (1 / 3)))), seed=100 + i)
+            d = cs.DeviceMatrix(ctx, cs.coo2dia(A.copy()), dtype=dt)
+            kind = "stencil"
+        else:
+            A = matrices.power_law(m, seed=100 + i, band=())
+            d = cs.DeviceMatrix(ctx, A.sorted(), panel=cs.PANEL_AUTO, dtype=dt)
+            kind = "powerlaw"
+        dev.append(d)
+        ids.append(server.add(d, 0, kmax=16))
+        kinds.append((kind, "bf16" if bf16 else "f32", A.height, A.nnz))
+    print(f"[bench] serve: {nmat} matrices resident ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    n = args.requests
+    mat = rng.integers(0, nmat, size=n)
+    ks = rng.integers(1, 17, size=n)
+    nnz_of = np.array([kk[3] for kk in kinds])
+
+    def trace(rate):
+        t = np.cumsum(rng.exponential(1.0 / rate, size=n)) if rate else np.zeros(n)
+        return [(ids[mat[i]], int(ks[i]), 10_000 + i, float(t[i]), None) for i in range(n)]
+
+    # Capacity: every request waiting at t = 0.
+    lat0, wall0 = server.replay(trace(None), mode=0)
+    cap = n / wall0
+    # Plain per-request launches (one at a time, each synchronised), on a
+    # sample of the same requests.
+    nb = min(n, args.baseline_requests)
+    latb, wallb = server.replay(trace(None)[:nb], mode=1)
+    plain = nb / wallb
+    # Poisson arrivals at a fraction of capacity.
+    rate = args.load * cap
+    lat, wall = server.replay(trace(rate), mode=0)
+    server.close()
+    served = n / wall
+    flops = 2.0 * float(np.sum(nnz_of[mat] * ks))
+    out = {
+        "metric": "served requests/s + p99 latency",
+        "value": round(served, 1), "unit": "requests/s", "n_gpus": 1, "steps": n, "warmup": 0,
+        "ms_per_step": round(wall * 1e3 / n, 6), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
+        "config": {"workload": f"{nmat} resident matrices (stencils + power-law, 10k..2M rows, FP32/BF16), "
+                               f"{n} Poisson requests, k in 1..16",
+                   "arrival_rate": round(rate, 1), "streams": args.streams},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                       "max": round(float(lat.max()) * 1e3, 3)},
+        "capacity_requests_per_s": round(cap, 1),
+        "plain_per_request_requests_per_s": round(plain, 1),
+        "speedup_vs_plain": round(cap / plain, 2),
+        "plain_latency_ms_p50": round(float(np.percentile(latb, 50)) * 1e3, 3),
+        "gflops_served": round(flops / wall / 1e9, 2),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="powerlaw", choices=["powerlaw", "stencil27", "lap2d", "block4x4"])
+    ap.add_argument("--workload", default="powerlaw",
+                    choices=["powerlaw", "stencil27", "lap2d", "block4x4", "spmm", "serve"])
+    ap.add_argument("--dtype", default="bf16", choices=["bf16", "fp16"], help="spmm: value type")
+    ap.add_argument("--k", type=int, nargs="+", default=[8, 16, 32], help="spmm: right-hand sides")
+    ap.add_argument("--err-columns", type=int, default=4, help="spmm: columns checked against FP64")
+    ap.add_argument("--matrices", type=int, default=64, help="serve: resident matrices")
+    ap.add_argument("--requests", type=int, default=20000, help="serve: requests")
+    ap.add_argument("--baseline-requests", type=int, default=2000, help="serve: plain-launch sample")
+    ap.add_argument("--load", type=float, default=0.7, help="serve: Poisson rate / capacity")
+    ap.add_argument("--streams", type=int, default=8, help="serve: HIP streams")
     ap.add_argument("--tables", default=None, help="directory holding benchmark/*.ben")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the tables' methods and graph launches")
@@ -112,6 +275,11 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); nothing is measured on more "
               "GPUs in this build.", file=sys.stderr)
         return 2
+
+    if args.workload == "spmm":
+        return run_spmm(args)
+    if args.workload == "serve":
+        return run_serve(args)
 
     import numpy as np
     import clspmv_mic_amd as cs

@@ -242,6 +242,7 @@ _sig("clspmv_upload_csr_ex", C.c_int, VP, P(CsrMatrix), P(UploadOpts), P(VP))
 _sig("clspmv_upload_dia_ex", C.c_int, VP, P(DiaMatrix), P(UploadOpts), P(VP))
 _sig("clspmv_dmat_set_panel", C.c_int, VP, C.c_int)
 _sig("clspmv_upload_bcsr_lowp", C.c_int, VP, P(BcsrMatrix), C.c_int, P(VP))
+_sig("clspmv_spmv_multi", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int)
 _sig("clspmv_spmm", C.c_int, VP, VP, C.c_int, VP, C.c_int, VP, C.c_int)
 _sig("clspmv_convert_from_f32", C.c_int, PF, VP, C.c_size_t, C.c_int)
 _sig("clspmv_convert_to_f32", C.c_int, VP, PF, C.c_size_t, C.c_int)
@@ -755,6 +756,10 @@ class DeviceMatrix:
 
     def _refresh(self):
         check(lib.clspmv_dmat_get_info(self.h, C.byref(self.info)), "dmat_info")
+
+    def spmv_multi(self, k: int, X, Y, ldy: int):
+        """Y[:, q] += A X[:, q] for q < k; X row-major (cols*k), Y column-major."""
+        check(lib.clspmv_spmv_multi(self.ctx.h, self.h, k, X.p, Y.p, ldy), "spmv_multi")
 
     def set_panel(self, use: bool):
         check(lib.clspmv_dmat_set_panel(self.h, int(use)), "dmat_set_panel")

@@ -43,6 +43,9 @@ struct ValF32 {
     static __device__ __forceinline__ float4 ld4(const float* p) {
         return ld_stream(reinterpret_cast<const float4*>(p));
     }
+    static __device__ __forceinline__ float4 ld4c(const float* p) {  // cached load
+        return *reinterpret_cast<const float4*>(p);
+    }
 };
 
 template <class Cvt>
@@ -55,7 +58,12 @@ struct Val16 {
         return make_float2(Cvt::f((T)(w & 0xFFFFu)), Cvt::f((T)(w >> 16)));
     }
     static __device__ __forceinline__ float4 ld4(const T* p) {
-        const u32x2 w = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+        return unpack(__builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p)));
+    }
+    static __device__ __forceinline__ float4 ld4c(const T* p) {  // cached load
+        return unpack(*reinterpret_cast<const u32x2*>(p));
+    }
+    static __device__ __forceinline__ float4 unpack(u32x2 w) {
         return make_float4(Cvt::f((T)(w.x & 0xFFFFu)), Cvt::f((T)(w.x >> 16)),
                            Cvt::f((T)(w.y & 0xFFFFu)), Cvt::f((T)(w.y >> 16)));
     }

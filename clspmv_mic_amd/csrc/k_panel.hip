@@ -13,10 +13,10 @@
 // products and reduces every segment (one row's run inside the sub-chunk)
 // with a segmented scan over the wave's lanes (DPP row shifts and
 // broadcasts); each segment's sum goes to seg_val.  Pass 2: one work-group
-// per row block loads the block's segments (one contiguous range, ordered
-// by panel then row), adds them to per-row LDS accumulators panel by panel,
-// and adds the totals to y.  The summation order is fixed by the encoding;
-// no atomics.
+// per row block loads the block's segments (one contiguous range), puts
+// each at its slot of a row-major LDS layout, and each row adds its slots
+// in panel order to y.  The summation order is fixed by the encoding; no
+// atomics.
 
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -28,7 +28,7 @@ using namespace dev;
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kU = 2;  // sub-chunks in flight per wave
+constexpr int kU = 4;  // sub-chunks in flight per wave
 
 // DPP controls (GFX9 encoding): shifts within 16-lane rows, broadcasts of
 // lane 15 / 31 to the following rows, and a whole-wave shift by one.
@@ -131,11 +131,17 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
     const int* __restrict__ sc_seg, const int* __restrict__ sc_start,
     const float* __restrict__ x, bool xa, float* __restrict__ seg_val) {
     extern __shared__ float xs[];
+    // The work-group's sub-chunk table, read once (loop indices then come
+    // from LDS instead of one dependent memory load per sub-chunk).
+    __shared__ int s_start[launch::kPanelScPerWg + 1];
+    __shared__ int s_seg[launch::kPanelScPerWg];
     const int4 t = wg[blockIdx.x];
     const int panel = t.x, sc0 = t.y, sc1 = t.z;
     const int xb = panel * pw;
     const int w = min(pw, cols - xb);
     const int tid = threadIdx.x;
+    if (tid <= sc1 - sc0) s_start[tid] = sc_start[sc0 + tid];
+    if (tid < sc1 - sc0) s_seg[tid] = sc_seg[sc0 + tid];
     // Stage the panel: every load is issued before the first LDS store, so
     // the work-group waits for one memory latency, not one per step.
     constexpr int kStage = (launch::kPanelMax + 4 * NT - 1) / (4 * NT);
@@ -185,10 +191,12 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
             q[u].c = u32x2{0x7FFF7FFFu, 0x7FFF7FFFu};
             q[u].v = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             if (c < sc1) {
-                const int i0 = sc_start[c] + lane * 4;
-                if (i0 < sc_start[c + 1]) {
-                    q[u].c = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(pcol + i0));
-                    const float4 v = V::ld4(pval + i0);
+                const int i0 = s_start[c - sc0] + lane * 4;
+                if (i0 < s_start[c - sc0 + 1]) {
+                    // Plain (cached) loads: measured faster here than
+                    // non-temporal ones, which the x panel does not need.
+                    q[u].c = *reinterpret_cast<const u32x2*>(pcol + i0);
+                    const float4 v = V::ld4c(pval + i0);
                     q[u].v = f32x4{v.x, v.y, v.z, v.w};
                 }
             }
@@ -201,7 +209,7 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int c = sc + u * nw;
-            if (c < sc1) seg_scan_write(cur[u], xs, lane, seg_val + sc_seg[c]);
+            if (c < sc1) seg_scan_write(cur[u], xs, lane, seg_val + s_seg[c - sc0]);
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
@@ -209,44 +217,37 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
 }
 
 __global__ __launch_bounds__(launch::kBlockThreads) void panel_pass2_kernel(
-    int npanels, const int* __restrict__ blk_row, const int* __restrict__ blk_tab,
-    const int* __restrict__ seg_row, const float* __restrict__ seg_val,
-    float* __restrict__ y) {
+    const int* __restrict__ blk_row, const int* __restrict__ blk_seg,
+    const int* __restrict__ row_slot, const unsigned short* __restrict__ seg_dst,
+    const float* __restrict__ seg_val, float* __restrict__ y) {
     constexpr int NT = launch::kBlockThreads, K = launch::kBlockItems;
-    __shared__ float acc[launch::kBlockRowsMax];
+    __shared__ float slot[launch::kBlockSegsMax];
     const int b = blockIdx.x, tid = threadIdx.x;
     const int row0 = blk_row[b], nrows = blk_row[b + 1] - row0;
-    const int* tab = blk_tab + (size_t)b * (npanels + 1);
-    const int j0 = tab[0], j1 = tab[npanels];
-    // All of the block's segments are loaded before any is used.
-    int r[K];
+    const int j0 = blk_seg[b], j1 = blk_seg[b + 1];
+    // Every segment of the block is loaded before any is placed.
+    unsigned short d[K];
     float v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int j = j0 + tid + k * NT;
-        r[k] = j < j1 ? seg_row[j] : -1;
+        d[k] = j < j1 ? seg_dst[j] : 0;
         v[k] = j < j1 ? seg_val[j] : 0.0f;
     }
-    // -0.0 is the identity of +: rows without segments keep y bit for bit.
-    for (int i = tid; i < nrows; i += NT) acc[i] = -0.0f;
-    __syncthreads();
-    for (int p = 0; p < npanels; ++p) {
-        const int lo = tab[p], hi = tab[p + 1];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int j = j0 + tid + k * NT;
-            // A continuation segment (bit 31) is added by its run's head;
-            // bit 30 tells the head that one follows.
-            if (j >= lo && j < hi && r[k] >= 0) {
-                float sum = v[k];
-                if (r[k] & 0x40000000)
-                    for (int q = j + 1; q < hi && seg_row[q] < 0; ++q) sum += seg_val[q];
-                acc[(r[k] & 0x3FFFFFFF) - row0] += sum;
-            }
-        }
-        __syncthreads();
+    for (int k = 0; k < K; ++k)
+        if (j0 + tid + k * NT < j1) slot[d[k]] = v[k];
+    __syncthreads();
+    // Each row adds its slots in order (panel order; a run cut at a
+    // sub-chunk boundary keeps its pieces in order).
+    const int* rs = row_slot + row0 + b;
+    for (int i = tid; i < nrows; i += NT) {
+        const int s = rs[i], e = rs[i + 1];
+        if (s == e) continue;
+        float acc = slot[s];
+        for (int q = s + 1; q < e; ++q) acc += slot[q];
+        y[row0 + i] += acc;
     }
-    for (int i = tid; i < nrows; i += NT) y[row0 + i] += acc[i];
 }
 
 }  // namespace
@@ -271,8 +272,8 @@ hipError_t panel_prepare() {
 hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
                       const unsigned short* pcol, int dtype, const void* pval,
                       const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
-                      float* seg_val, int npanels, int nblocks, const int* blk_row,
-                      const int* blk_tab, const int* seg_row, float* y) {
+                      float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
+                      const int* row_slot, const unsigned short* seg_dst, float* y) {
     if (nwg <= 0 || nblocks <= 0) return hipSuccess;
     if (pw <= 0 || pw > kPanelMax) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * (size_t)pw;
@@ -297,8 +298,8 @@ hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(panel_pass2_kernel, dim3(nblocks), dim3(kBlockThreads), 0, s, npanels,
-                       blk_row, blk_tab, seg_row, seg_val, y);
+    hipLaunchKernelGGL(panel_pass2_kernel, dim3(nblocks), dim3(kBlockThreads), 0, s, blk_row, blk_seg,
+                       row_slot, seg_dst, seg_val, y);
     return hipGetLastError();
 }
 

@@ -11,9 +11,11 @@
 //   B[k][j]: lane 4g+j, element k      -> X rows c0..c0+3 of column j
 //   D[i][j]: lane 4g+j, register i     -> Y rows r0..r0+3 of column j
 // so with X and Y column-major every lane moves whole 8- or 16-byte pieces.
-// A group owns one (block row, quad of right-hand sides); each Y element
-// has one owner, so Y is updated without atomics and the summation order
+// A group owns one block row and all right-hand sides; each Y element has
+// one owner, so Y is updated without atomics and the summation order
 // (blocks in storage order, four products per instruction) is fixed.
+
+#include <algorithm>
 
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -47,21 +49,19 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
     return v;
 }
 
-constexpr int kUnroll = 4;
-
-template <bool BF16>
+// A group of four lanes owns one block row and every right-hand side (in
+// quads of four): each A block is loaded once and multiplied by NQ tiles of
+// X, so A is read once per SpMM whatever k is.  Lane t of a group loads row
+// t of the block and, per quad q, column 4q + t of X.
+template <bool BF16, int NQ>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
-    int brows, int rows, int cols, int k, int nq, const int* __restrict__ rp,
+    int brows, int rows, int cols, int k, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
     const unsigned short* __restrict__ X, int ldx, float* __restrict__ Y, int ldy) {
     const int lane = threadIdx.x & (kWave - 1);
-    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) / kWave;
-    const long long g = wave * 16 + lane / 4;
+    const int br = (int)(((long long)blockIdx.x * kBlock + threadIdx.x) / 4);
     const int t = lane & 3;
-    const int br = (int)(g / nq), q = (int)(g % nq);
     const bool active = br < brows;
-    const int xc = q * 4 + t;  // right-hand side of this lane
-    const bool col_ok = active && xc < k;
     int j0 = 0, n = 0;
     if (active) {
         j0 = rp[br];
@@ -72,39 +72,86 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int nmax = n;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) nmax = max(nmax, __shfl_xor(nmax, m, kWave));
-    const unsigned short* xcol = X + (size_t)(col_ok ? xc : 0) * ldx;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int it = 0; it < nmax; it += kUnroll) {
-        s16x4 a[kUnroll], b[kUnroll];
+    f32x4 acc[NQ];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            a[u] = s16x4{0, 0, 0, 0};
-            b[u] = s16x4{0, 0, 0, 0};
-            if (it + u < n) {
-                const int j = j0 + it + u;
-                a[u] = *reinterpret_cast<const s16x4*>(val + (size_t)j * 16 + t * 4);
-                if (col_ok) {
-                    const int c0 = bcol[j];
-                    b[u] = ld4(xcol + c0, cols - c0);
-                }
-            }
+    for (int q = 0; q < NQ; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // Operands of block `it` (zeros past the group's own blocks).
+    auto fetch = [&](int it, s16x4& a, s16x4* b) {
+        a = s16x4{0, 0, 0, 0};
+        int c0 = 0;
+        const bool has = it < n;
+        if (has) {
+            const int j = j0 + it;
+            a = *reinterpret_cast<const s16x4*>(val + (size_t)j * 16 + t * 4);
+            c0 = bcol[j];
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) acc = mfma4<BF16>(a[u], b[u], acc);
+        for (int q = 0; q < NQ; ++q) {
+            const int xc = q * 4 + t;
+            b[q] = s16x4{0, 0, 0, 0};
+            if (has && xc < k) b[q] = ld4(X + (size_t)xc * ldx + c0, cols - c0);
+        }
+    };
+    // Software pipeline: block it + 1's loads are in flight while block it
+    // is multiplied.
+    s16x4 a_cur, a_nxt, b_cur[NQ], b_nxt[NQ];
+    fetch(0, a_cur, b_cur);
+    for (int it = 0; it < nmax; ++it) {
+        fetch(it + 1, a_nxt, b_nxt);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] = mfma4<BF16>(a_cur, b_cur[q], acc[q]);
+        a_cur = a_nxt;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) b_cur[q] = b_nxt[q];
     }
-    if (col_ok) {
-        const int r0 = br * 4;
+    if (!active) return;
+    const int r0 = br * 4;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int xc = q * 4 + t;
+        if (xc >= k) continue;
         float* yp = Y + (size_t)xc * ldy + r0;
         if (r0 + 3 < rows) {
             f32x4 y = *reinterpret_cast<const f32x4*>(yp);
-            y += acc;
+            y += acc[q];
             *reinterpret_cast<f32x4*>(yp) = y;
         } else {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (r0 + i < rows) yp[i] += acc[i];
+                if (r0 + i < rows) yp[i] += acc[q][i];
         }
     }
+}
+
+template <bool BF16, int NQ>
+hipError_t launch_nq(hipStream_t s, int brows, int rows, int cols, int k, const int* rp,
+                     const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
+                     float* Y, int ldy) {
+    const long long threads = (long long)brows * 4;
+    const long long blocks = (threads + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ>), dim3((unsigned)blocks), dim3(kBlock), 0, s, brows, rows,
+                       cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    return hipGetLastError();
+}
+
+template <bool BF16>
+hipError_t launch_k(hipStream_t s, int brows, int rows, int cols, int k, const int* rp,
+                    const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
+                    float* Y, int ldy) {
+    // Up to 32 right-hand sides per launch (8 quads of accumulators).
+    for (int c = 0; c < k; c += 32) {
+        const int kc = std::min(32, k - c);
+        const unsigned short* Xc = X + (size_t)c * ldx;
+        float* Yc = Y + (size_t)c * ldy;
+        const int nq = (kc + 3) / 4;
+        hipError_t e;
+        if (nq <= 1) e = launch_nq<BF16, 1>(s, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else if (nq <= 2) e = launch_nq<BF16, 2>(s, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else if (nq <= 4) e = launch_nq<BF16, 4>(s, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else e = launch_nq<BF16, 8>(s, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -115,18 +162,9 @@ hipError_t bspmm4(hipStream_t s, bool bf16, int brows, int rows, int cols, int k
                   const int* rp, const int* bcol, const unsigned short* val,
                   const unsigned short* X, int ldx, float* Y, int ldy) {
     if (brows <= 0 || k <= 0) return hipSuccess;
-    const int nq = (k + 3) / 4;
-    const long long groups = (long long)brows * nq;
-    const long long waves = (groups + 15) / 16;
-    const long long blocks = (waves * kWave + kBlock - 1) / kBlock;
-    if (blocks > 0x7FFFFFFFll) return hipErrorInvalidValue;
-    if (bf16)
-        hipLaunchKernelGGL(bspmm4_kernel<true>, dim3((unsigned)blocks), dim3(kBlock), 0, s, brows, rows,
-                           cols, k, nq, rp, bcol, val, X, ldx, Y, ldy);
-    else
-        hipLaunchKernelGGL(bspmm4_kernel<false>, dim3((unsigned)blocks), dim3(kBlock), 0, s, brows, rows,
-                           cols, k, nq, rp, bcol, val, X, ldx, Y, ldy);
-    return hipGetLastError();
+    if ((long long)brows * 4 / kBlock + 1 > 0x7FFFFFFFll) return hipErrorInvalidValue;
+    return bf16 ? launch_k<true>(s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy)
+                : launch_k<false>(s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
 }
 
 }  // namespace launch

@@ -42,24 +42,27 @@ hipError_t coo_segreduce(hipStream_t s, int nnz, const int* row, const int* col,
 // (panel, first sub-chunk, end sub-chunk, 0) per group) writes each
 // segment's sum to seg_val[sc_seg[sub-chunk] + k].  Segments are numbered
 // in (row block, panel, row) order, so pass 2 (one work-group per row block)
-// reads its block's segments as one contiguous range, adds them per row in
-// panel order in LDS, and adds each row's total to y.
+// reads its block's segments as one contiguous range, places each in LDS at
+// its precomputed slot (seg_dst) of the block's row-major layout (row_slot:
+// first slot per row), and adds each row's slots, in panel order, to y.
 constexpr int kPanelSub = 256;        // entries per wave-sized sub-chunk
 constexpr int kPanelMax = 32760;      // largest panel width (floats)
 constexpr unsigned kPanelPad = 0x7FFF;  // column code of padding entries
 constexpr int kPanelThreads = 1024;   // pass-1 work-group size
+constexpr int kPanelScPerWg = 128;    // most sub-chunks of one pass-1 work-group
+constexpr int kPanelsMax = 2048;      // most panels of one matrix (bounds the encoder's tables)
 constexpr int kBlockRowsMax = 4096;   // rows of a pass-2 row block
 constexpr int kBlockThreads = 1024;   // pass-2 work-group size
 constexpr int kBlockItems = 8;        // segments per pass-2 thread
-constexpr int kBlockSegsMax = kBlockThreads * kBlockItems;
+constexpr int kBlockSegsMax = kBlockThreads * kBlockItems;  // < 65536 (16-bit slots)
 // Raises pass 1's dynamic LDS limit on the current device (call before the
 // first launch, outside graph capture).
 hipError_t panel_prepare();
 hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
                       const unsigned short* pcol, int dtype, const void* pval,
                       const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
-                      float* seg_val, int npanels, int nblocks, const int* blk_row,
-                      const int* blk_tab, const int* seg_row, float* y);
+                      float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
+                      const int* row_slot, const unsigned short* seg_dst, float* y);
 
 // ---- ELL / SELL --------------------------------------------------------------
 hipError_t ell(hipStream_t s, int rows, int hal, int ellnum, int rows_per_lane,
@@ -93,6 +96,14 @@ hipError_t bell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
 hipError_t sbell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
                  int slice_height, const int* sptr, const int* col4,
                  const float* data, const float* x, bool x_aligned, float* y);
+
+// ---- multi-right-hand-side SpMV (k <= kMultiMax) --------------------------------
+// Y[:, q] += A X[:, q]; X row-major (X[c*k + q]), Y column-major (ldy).
+constexpr int kMultiMax = 16;
+hipError_t dia_multi(hipStream_t s, int rows, int cols, int len, int ndiag, const int* offsets,
+                     int dtype, const void* data, int k, const float* X, float* Y, int ldy);
+hipError_t csr_multi(hipStream_t s, int rows, int lanes, const int* rp, const int* col, int dtype,
+                     const void* val, int k, const float* X, float* Y, int ldy);
 
 // ---- mixed-precision 4x4-block SpMM (MFMA) -------------------------------------
 // Y[:, 0:k] += A X[:, 0:k]; A: block rows rp/bcol (element columns, multiples

@@ -28,6 +28,9 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
                     launch::kPanelMax);
     if (cols <= 0 || rows <= 0 || nnz <= 0) return CLSPMV_OK;
     const int P = (cols + pw - 1) / pw;
+    if (P > launch::kPanelsMax)
+        return fail(CLSPMV_ERR_UNSUPPORTED, "%d panels of width %d: at most %d panels", P, pw,
+                    launch::kPanelsMax);
 
     // Row blocks for pass 2: at most kBlockRowsMax rows and, bounding each
     // row's segments by its per-panel runs (a run of L > 1 entries becomes
@@ -149,6 +152,26 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
         }
     }
 
+    // Pass 2 gathers each block's segments row by row: segment id -> slot
+    // in the block's row-major layout (rows ascending, each row's segments
+    // in id order, i.e. panel order with continuations after their head),
+    // and per block the first slot of each row.
+    std::vector<unsigned short> seg_dst(nseg);
+    std::vector<int> blk_seg(nb + 1);
+    std::vector<int> row_slot((size_t)rows + nb);
+    for (int b = 0; b < nb; ++b) {
+        const int j0 = blk_tab[(size_t)b * (P + 1)], j1 = blk_tab[(size_t)b * (P + 1) + P];
+        const int r0 = blk_row[b], nr = blk_row[b + 1] - r0;
+        blk_seg[b] = j0;
+        int* slot = &row_slot[(size_t)r0 + b];  // nr + 1 entries
+        std::fill(slot, slot + nr + 1, 0);
+        for (int j = j0; j < j1; ++j) slot[(seg_row[j] & 0x3FFFFFFF) - r0 + 1]++;
+        for (int i = 0; i < nr; ++i) slot[i + 1] += slot[i];
+        std::vector<int> cur(slot, slot + nr);
+        for (int j = j0; j < j1; ++j) seg_dst[j] = (unsigned short)cur[(seg_row[j] & 0x3FFFFFFF) - r0]++;
+    }
+    blk_seg[nb] = (int)nseg;
+
     // What the kernels rely on: every sub-chunk opens a segment, its
     // segments' slots exist, and its lanes read whole 4-entry groups; each
     // block's segments fit one pass-2 work-group and belong to its rows.
@@ -173,7 +196,7 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
     }
 
     // Pass-1 work-groups: runs of at most W sub-chunks of one panel.
-    const int W = std::max(32, std::min(128, pw / kSub));
+    const int W = std::max(32, std::min(launch::kPanelScPerWg, pw / kSub));
     std::vector<int> wg;
     for (int p = 0; p < P; ++p) {
         const int b0 = g_sc[(size_t)p * nb], b1 = g_sc[(size_t)(p + 1) * nb];
@@ -206,9 +229,10 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
     if (!st) st = upload_array(d, sc_seg.data(), sc_seg.size(), &d->psc_seg);
     if (!st) st = upload_array(d, sc_start.data(), sc_start.size(), &d->psc_start);
     if (!st) st = upload_array<float>(d, nullptr, (size_t)nseg, &d->pseg);
-    if (!st) st = upload_array(d, seg_row.data(), seg_row.size(), &d->pseg_row);
+    if (!st) st = upload_array(d, seg_dst.data(), seg_dst.size(), &d->pseg_dst);
     if (!st) st = upload_array(d, blk_row.data(), blk_row.size(), &d->pblk_row);
-    if (!st) st = upload_array(d, blk_tab.data(), blk_tab.size(), &d->pblk_tab);
+    if (!st) st = upload_array(d, blk_seg.data(), blk_seg.size(), &d->pblk_seg);
+    if (!st) st = upload_array(d, row_slot.data(), row_slot.size(), &d->prow_slot);
     if (st) return st;
     d->use_panel = true;
     return CLSPMV_OK;
@@ -224,9 +248,10 @@ void drop_encoding(clspmv_dmat* d, bool panel) {
         release_array(d, &d->psc_seg);
         release_array(d, &d->psc_start);
         release_array(d, &d->pseg);
-        release_array(d, &d->pseg_row);
+        release_array(d, &d->pseg_dst);
         release_array(d, &d->pblk_row);
-        release_array(d, &d->pblk_tab);
+        release_array(d, &d->pblk_seg);
+        release_array(d, &d->prow_slot);
         d->pw = 0;
         d->use_panel = false;
         return;

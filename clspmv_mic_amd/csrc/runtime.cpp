@@ -111,7 +111,7 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
         e = launch::panel_spmv(s, A->pnwg, A->pwg, A->pw, I.cols, A->pcol, A->dtype,
                                A->dtype ? (const void*)A->plpval : (const void*)A->pval, A->psc_seg,
                                A->psc_start, x, xa,
-                               A->pseg, A->pnp, A->pnb, A->pblk_row, A->pblk_tab, A->pseg_row, y);
+                               A->pseg, A->pnb, A->pblk_row, A->pblk_seg, A->prow_slot, A->pseg_dst, y);
         if (e != hipSuccess)
             return fail(CLSPMV_ERR_HIP, "panel launch failed (format %d): %s", I.format,
                         hipGetErrorString(e));
@@ -119,6 +119,10 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
     }
     switch (I.format) {
         case CLSPMV_FMT_CSR:
+            if (A->dtype) {  // 16-bit values: the multi-vector kernel with k = 1
+                e = launch::csr_multi(s, I.rows, A->lanes, A->rp, A->col, A->dtype, A->lpval, 1, x, y, I.rows);
+                break;
+            }
             e = method == 0 ? launch::csr_vector(s, I.rows, A->lanes, A->rp, A->col, A->val, x, y)
                             : launch::csr_adaptive(s, A->nblocks, A->blocks, A->rp, A->col, A->val, x, y);
             break;
@@ -179,6 +183,26 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
     if (e != hipSuccess)
         return fail(CLSPMV_ERR_HIP, "launch failed (format %d method %d): %s", I.format,
                     method, hipGetErrorString(e));
+    return CLSPMV_OK;
+}
+
+bool has_multi(const clspmv_dmat* A) {
+    return A->info.format == CLSPMV_FMT_DIA || (A->info.format == CLSPMV_FMT_CSR && A->plain);
+}
+
+int multi_launch(hipStream_t s, const clspmv_dmat* A, int k, const float* X, float* Y, int ldy) {
+    const clspmv_dmat_info& I = A->info;
+    hipError_t e;
+    if (I.format == CLSPMV_FMT_DIA) {
+        e = launch::dia_multi(s, std::min(I.rows, A->len), I.cols, A->len, A->ndiag, A->offs, A->dtype,
+                              A->dtype ? (const void*)A->lpval : (const void*)A->val, k, X, Y, ldy);
+    } else if (I.format == CLSPMV_FMT_CSR && A->plain) {
+        e = launch::csr_multi(s, I.rows, A->lanes, A->rp, A->col, A->dtype,
+                              A->dtype ? (const void*)A->lpval : (const void*)A->val, k, X, Y, ldy);
+    } else {
+        return fail(CLSPMV_ERR_UNSUPPORTED, "multi-vector SpMV exists for DIA and CSR (own arrays) only");
+    }
+    if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "multi-vector launch failed: %s", hipGetErrorString(e));
     return CLSPMV_OK;
 }
 
@@ -342,12 +366,12 @@ static int add_panel(clspmv_dmat* d, const clspmv_upload_opts& o, const int* row
     return CLSPMV_OK;
 }
 
-static int check_opts(const clspmv_upload_opts& o) {
+static int check_opts(const clspmv_upload_opts& o, bool csr) {
     if (o.dtype != CLSPMV_DT_F32 && o.dtype != CLSPMV_DT_BF16 && o.dtype != CLSPMV_DT_F16)
         return fail(CLSPMV_ERR_INVALID, "dtype %d", o.dtype);
     const int pw = o.panel_width;
-    if (o.dtype != CLSPMV_DT_F32 && (pw == 0 || pw == CLSPMV_PANEL_BOTH))
-        return fail(CLSPMV_ERR_UNSUPPORTED, "16-bit COO / CSR values need the panel encoding only");
+    if (o.dtype != CLSPMV_DT_F32 && (pw == CLSPMV_PANEL_BOTH || (pw == 0 && !csr)))
+        return fail(CLSPMV_ERR_UNSUPPORTED, "16-bit COO values need the panel encoding only");
     if (pw == 0 || pw == CLSPMV_PANEL_AUTO || pw == CLSPMV_PANEL_BOTH) return CLSPMV_OK;
     if (pw < 8 || pw % 8 || pw > launch::kPanelMax)
         return fail(CLSPMV_ERR_INVALID, "panel width %d: need 0, -1, -2 or a multiple of 8 in [8, %d]", pw,
@@ -367,7 +391,7 @@ int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m,
                          const clspmv_upload_opts* opts, clspmv_dmat** out) {
     const clspmv_upload_opts o = opts ? *opts : clspmv_upload_opts{0, CLSPMV_DT_F32};
     {
-        int st0 = check_opts(o);
+        int st0 = check_opts(o, true);
         if (st0) return st0;
     }
     UPLOAD_PROLOGUE(m, CLSPMV_FMT_CSR)
@@ -389,7 +413,15 @@ int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m,
     d->lanes = pow2_at_least((int)std::ceil(avg), 64);
     UPLOAD_CHECK(upload_array(d, m->csr_row_ptr, (size_t)rows + 1, &d->rp));
     UPLOAD_CHECK(upload_array(d, m->csr_col_id, nnz, &d->col));
-    UPLOAD_CHECK(upload_array(d, m->csr_data, nnz, &d->val));
+    if (o.dtype != CLSPMV_DT_F32 && o.panel_width == 0) {
+        // Own arrays with 16-bit values (run by the multi-vector kernel).
+        std::vector<unsigned short> lp(nnz);
+        UPLOAD_CHECK(clspmv_convert_from_f32(m->csr_data, lp.data(), (size_t)nnz, o.dtype));
+        UPLOAD_CHECK(upload_array(d, lp.data(), lp.size(), &d->lpval));
+        d->dtype = o.dtype;
+    } else {
+        UPLOAD_CHECK(upload_array(d, m->csr_data, nnz, &d->val));
+    }
     std::vector<int> blocks = adaptive_blocks(m->csr_row_ptr, rows);
     d->nblocks = (int)blocks.size() - 1;
     UPLOAD_CHECK(upload_array(d, blocks.data(), blocks.size(), &d->blocks));
@@ -407,7 +439,7 @@ int clspmv_upload_coo_ex(clspmv_ctx* ctx, const clspmv_coo_matrix* m,
                          const clspmv_upload_opts* opts, clspmv_dmat** out) {
     const clspmv_upload_opts o = opts ? *opts : clspmv_upload_opts{CLSPMV_PANEL_AUTO, CLSPMV_DT_F32};
     {
-        int st0 = check_opts(o);
+        int st0 = check_opts(o, false);
         if (st0) return st0;
     }
     UPLOAD_PROLOGUE(m, CLSPMV_FMT_COO)
@@ -759,6 +791,18 @@ int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, i
                                   d_Y, ldy);
     if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "spmm launch failed: %s", hipGetErrorString(e));
     return CLSPMV_OK;
+}
+
+int clspmv_spmv_multi(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const float* d_X, float* d_Y,
+                      int ldy) {
+    if (!ctx || !A || (k > 0 && (!d_X || !d_Y))) return fail(CLSPMV_ERR_INVALID, "null argument");
+    const clspmv_dmat_info& I = A->info;
+    if (k < 0 || k > launch::kMultiMax) return fail(CLSPMV_ERR_INVALID, "k = %d outside 0..%d", k, launch::kMultiMax);
+    if (ldy < I.rows) return fail(CLSPMV_ERR_INVALID, "ldy %d < rows %d", ldy, I.rows);
+    int st = use_ctx_device(ctx);
+    if (st) return st;
+    if (k == 0 || I.rows == 0 || I.cols == 0 || I.nnz == 0) return CLSPMV_OK;
+    return multi_launch(ctx->stream, A, k, d_X, d_Y, ldy);
 }
 
 int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out) {

@@ -52,9 +52,10 @@ struct clspmv_dmat {
     int* psc_seg = nullptr;
     int* psc_start = nullptr;  // first entry of each sub-chunk (+ end)
     float* pseg = nullptr;
-    int* pseg_row = nullptr;   // row | continuation bit, per segment
+    unsigned short* pseg_dst = nullptr;  // slot of each segment in its block
     int* pblk_row = nullptr;   // first row of each row block (+ end)
-    int* pblk_tab = nullptr;   // first segment of (block, panel)
+    int* pblk_seg = nullptr;   // first segment of each row block (+ end)
+    int* prow_slot = nullptr;  // per block: first slot of each row (+ end)
     bool use_panel = false;
     bool plain = true;
     // Values stored in 16 bits (dtype CLSPMV_DT_BF16 / CLSPMV_DT_F16): DIA
@@ -128,6 +129,11 @@ int use_ctx_device(clspmv_ctx* ctx);
 // beyond the method's shape requirements).
 int dmat_launch(hipStream_t stream, const clspmv_dmat* A, int method,
                 const float* x, float* y);
+
+// Multi-vector SpMV (DIA, CSR with its own arrays): Y[:, q] += A X[:, q],
+// X row-major, Y column-major; no argument checks.
+bool has_multi(const clspmv_dmat* A);
+int multi_launch(hipStream_t s, const clspmv_dmat* A, int k, const float* X, float* Y, int ldy);
 
 // Method chosen for CLSPMV_METHOD_AUTO.
 int auto_method(const clspmv_dmat* A);

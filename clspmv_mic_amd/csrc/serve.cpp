@@ -6,7 +6,8 @@
 // in-flight HBM budget -- skipping busy matrices rather than blocking on
 // them -- and (3) launches each admitted request on its matrix's stream:
 // generate x from the seed, then replay the captured (matrix, k) graph
-// (zero y, k SpMVs or one SpMM).  Callers only touch the queue.
+// (zero y, then one multi-vector pass, one SpMM, or k SpMVs).  Callers only
+// touch the queue.
 
 #include <algorithm>
 #include <chrono>
@@ -55,6 +56,16 @@ __global__ void fill_x_kernel(unsigned long long seed, int k, int cols, int ldx,
     }
 }
 
+// Row-major x for the multi-vector kernels: X[i*k + j] = x_j[i].
+__global__ void fill_x_rows_kernel(unsigned long long seed, int k, int cols, float* X) {
+    const long long n = (long long)k * cols;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t / k), j = (int)(t % k);
+        X[t] = uniform_at(seed, j, i);
+    }
+}
+
 __global__ void fill_x16_kernel(unsigned long long seed, int k, int cols, int ldx, int dtype,
                                 unsigned short* X) {
     const long long n = (long long)k * cols;
@@ -85,6 +96,7 @@ struct SMatrix {
     const clspmv_dmat* A = nullptr;
     int method = 0, rows = 0, cols = 0, kmax = 0, ldx = 0, ldy = 0, stream = 0;
     bool lowp = false;
+    bool multi = false;  // k > 1 runs one multi-vector pass (X row-major)
     void* X = nullptr;
     float* Y = nullptr;
     float* pinned = nullptr;  // rows * kmax staging for host copies
@@ -130,8 +142,9 @@ size_t request_bytes(const SMatrix& M, int k) {
     return (size_t)k * ((size_t)M.ldx * (M.lowp ? 2 : 4) + (size_t)M.ldy * 4);
 }
 
-// Zero y, then the k products.  Runs on `s` (captured or not).
-int enqueue_pass(hipStream_t s, const SMatrix& M, int k) {
+// Zero y, then the k products (one multi-vector pass when `multi`, else one
+// SpMV per right-hand side).  Runs on `s` (captured or not).
+int enqueue_pass(hipStream_t s, const SMatrix& M, int k, bool multi) {
     const long long ny = (long long)M.ldy * k;
     hipLaunchKernelGGL(zero_kernel, dim3(grid_for(ny)), dim3(256), 0, s, ny, M.Y);
     CLSPMV_HIP(hipGetLastError());
@@ -141,6 +154,7 @@ int enqueue_pass(hipStream_t s, const SMatrix& M, int k) {
                                   static_cast<const unsigned short*>(M.X), M.ldx, M.Y, M.ldy));
         return CLSPMV_OK;
     }
+    if (multi && M.multi && k > 1) return multi_launch(s, M.A, k, static_cast<const float*>(M.X), M.Y, M.ldy);
     for (int j = 0; j < k; ++j) {
         int st = dmat_launch(s, M.A, M.method, static_cast<const float*>(M.X) + (size_t)j * M.ldx,
                              M.Y + (size_t)j * M.ldy);
@@ -149,11 +163,14 @@ int enqueue_pass(hipStream_t s, const SMatrix& M, int k) {
     return CLSPMV_OK;
 }
 
-int enqueue_x(hipStream_t s, const SMatrix& M, int k, unsigned long long seed) {
+int enqueue_x(hipStream_t s, const SMatrix& M, int k, unsigned long long seed, bool multi) {
     const long long n = (long long)k * M.cols;
     if (M.lowp)
         hipLaunchKernelGGL(fill_x16_kernel, dim3(grid_for(n)), dim3(256), 0, s, seed, k, M.cols, M.ldx,
                            M.A->dtype, static_cast<unsigned short*>(M.X));
+    else if (multi && M.multi && k > 1)
+        hipLaunchKernelGGL(fill_x_rows_kernel, dim3(grid_for(n)), dim3(256), 0, s, seed, k, M.cols,
+                           static_cast<float*>(M.X));
     else
         hipLaunchKernelGGL(fill_x_kernel, dim3(grid_for(n)), dim3(256), 0, s, seed, k, M.cols, M.ldx,
                            static_cast<float*>(M.X));
@@ -165,7 +182,7 @@ int graph_for(clspmv_server* S, SMatrix& M, int k, hipGraphExec_t* out) {
     if (!M.exec[k]) {
         hipStream_t s = S->streams[M.stream];
         CLSPMV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int st = enqueue_pass(s, M, k);
+        int st = enqueue_pass(s, M, k, true);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s, &g);
         if (st) {
@@ -190,13 +207,13 @@ int copy_out(hipStream_t s, const SMatrix& M, int k, float* dst) {
 int launch_request(clspmv_server* S, SRequest& q) {
     SMatrix& M = S->mats[q.r.matrix];
     hipStream_t s = S->streams[M.stream];
-    int st = enqueue_x(s, M, q.r.k, q.r.seed);
+    int st = enqueue_x(s, M, q.r.k, q.r.seed, true);
     if (st) return st;
     if (S->cfg.use_graphs) {
         hipGraphExec_t ex = nullptr;
         if ((st = graph_for(S, M, q.r.k, &ex))) return st;
         CLSPMV_HIP(hipGraphLaunch(ex, s));
-    } else if ((st = enqueue_pass(s, M, q.r.k))) {
+    } else if ((st = enqueue_pass(s, M, q.r.k, true))) {
         return st;
     }
     if (q.r.host_y && (st = copy_out(s, M, q.r.k, M.pinned))) return st;
@@ -281,11 +298,13 @@ void scheduler(clspmv_server* S) {
     }
 }
 
+// The baseline: one request at a time, one SpMV launch per right-hand side
+// (as a caller of clspmv_spmv would issue them), synchronised at the end.
 int run_plain(clspmv_server* S, const clspmv_request& r) {
     SMatrix& M = S->mats[r.matrix];
     hipStream_t s = S->streams[0];
-    int st = enqueue_x(s, M, r.k, r.seed);
-    if (!st) st = enqueue_pass(s, M, r.k);
+    int st = enqueue_x(s, M, r.k, r.seed, false);
+    if (!st) st = enqueue_pass(s, M, r.k, false);
     if (!st && r.host_y) st = copy_out(s, M, r.k, M.pinned);
     if (!st) CLSPMV_HIP(hipStreamSynchronize(s));
     if (!st && r.host_y) std::copy(M.pinned, M.pinned + (size_t)M.rows * r.k, r.host_y);
@@ -366,6 +385,7 @@ int clspmv_server_add_matrix(clspmv_server* S, const clspmv_dmat* A, int method,
     SMatrix M;
     M.A = A;
     M.lowp = A->info.format == CLSPMV_FMT_BSPMM;
+    M.multi = !M.lowp && has_multi(A);
     M.method = M.lowp ? 0 : (method == CLSPMV_METHOD_AUTO ? auto_method(A) : method);
     if (!M.lowp && (M.method < 0 || M.method >= clspmv_method_count(A->info.format)))
         return fail(CLSPMV_ERR_INVALID, "method %d not available for format %d", method, A->info.format);

@@ -96,8 +96,8 @@ def power_law(m: int = 1_000_000, seed: int = 2, s: float = 1.8, max_len: int = 
         r = np.arange(max(0, -off), min(m, m - off), dtype=np.int64)
         brow.append(r)
         bcol.append(r + off)
-    brow = np.concatenate(brow)
-    bcol = np.concatenate(bcol)
+    brow = np.concatenate(brow) if brow else np.zeros(0, np.int64)
+    bcol = np.concatenate(bcol) if bcol else np.zeros(0, np.int64)
     bval = rng.standard_normal(brow.size).astype(np.float32)
     # Band entries first so that they win over random duplicates.
     return _finish(np.concatenate([brow, rows]).astype(np.int32),

@@ -124,8 +124,9 @@ int clspmv_dmat_free(clspmv_dmat* m);
  * dtype: CLSPMV_DT_F32 (the reference's values), or CLSPMV_DT_BF16 /
  * CLSPMV_DT_F16: values rounded to 16 bits (nearest even) and widened to
  * FP32 in the kernels, which accumulate in FP32 as before.  16-bit values
- * exist for DIA (every method) and for the COO / CSR panel encoding (AUTO
- * then always uses panels; 0 and BOTH are refused).
+ * exist for DIA (every method), for the COO / CSR panel encoding (AUTO then
+ * always uses panels; BOTH is refused) and for CSR's own arrays
+ * (panel_width 0; every method runs the multi-vector kernel with k = 1).
  *
  * clspmv_upload_coo is clspmv_upload_coo_ex with {CLSPMV_PANEL_AUTO, F32},
  * clspmv_upload_csr is clspmv_upload_csr_ex with {0, F32}, clspmv_upload_dia
@@ -152,6 +153,15 @@ int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out);
  * y holds rows floats.  method = CLSPMV_METHOD_AUTO picks by shape. */
 int clspmv_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
                 const float* d_x, float* d_y);
+
+/* Multi-vector SpMV (no counterpart in the reference): Y[:, q] += A X[:, q]
+ * for q < k (k <= 16) in one pass over the matrix.  X is row-major: the k
+ * values of column c are X[c*k .. c*k+k) (cols * k floats), so one gather
+ * serves every right-hand side; Y is column-major with leading dimension
+ * ldy >= rows.  DIA matrices (any value type) and CSR matrices holding
+ * their own arrays (FP32 or 16-bit values) only.  Asynchronous. */
+int clspmv_spmv_multi(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const float* d_X,
+                      float* d_Y, int ldy);
 
 /* Times ntimes back-to-back launches of clspmv_spmv after warmup
  * untimed ones, with HIP events on the context's stream; y keeps

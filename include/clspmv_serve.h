@@ -10,8 +10,9 @@
  *     request whose matrix is idle and whose buffers fit the HBM budget,
  *     while earlier requests are still running; requests run concurrently
  *     on several HIP streams;
- *   - each (matrix, k) pass is captured once into a HIP graph (zero y, k
- *     SpMVs or one SpMM) and replayed for every later request;
+ *   - each (matrix, k) pass is captured once into a HIP graph (zero y, then
+ *     one multi-vector pass, one SpMM, or k SpMVs) and replayed for every
+ *     later request;
  *   - results do not depend on the mix: every kernel has a fixed summation
  *     order and a request only touches its matrix's own buffers.
  *
@@ -49,7 +50,9 @@ int clspmv_server_destroy(clspmv_server* s);
  * server) and allocates its request buffers for up to kmax right-hand
  * sides.  FP32 matrices run `method` (clspmv_spmv numbering) once per
  * right-hand side; CLSPMV_FMT_BSPMM matrices run clspmv_spmm with x rounded
- * to the matrix's 16-bit type.  *id receives the matrix id. */
+ * to the matrix's 16-bit type.  DIA and CSR matrices run requests with
+ * k > 1 as one multi-vector pass (clspmv_spmv_multi) instead.  *id receives
+ * the matrix id. */
 int clspmv_server_add_matrix(clspmv_server* s, const clspmv_dmat* A, int method, int kmax,
                              int* id);
 
@@ -72,9 +75,9 @@ double clspmv_server_clock(clspmv_server* s);
  * server clock, relative to the start of the replay) and the call returns
  * when all are done, with each request's latency and the wall time.
  * mode 0: the serving loop.  mode 1: plain per-request launches, the
- * baseline -- the calling thread runs each request alone (generate x, zero
- * y, launch the kernels, copy y out if asked, synchronize) in arrival
- * order. */
+ * baseline -- the calling thread runs each request alone in arrival order:
+ * generate x, zero y, one SpMV (or SpMM) launch per right-hand side as a
+ * caller of clspmv_spmv would, copy y out if asked, synchronize. */
 int clspmv_server_replay(clspmv_server* s, int n, const clspmv_request* reqs, int mode,
                          double* latencies, double* wall_s);
 

@@ -226,6 +226,11 @@ def test_panel_encoding(ctx, oracle, name, pw, fmt):
     one to many panels, COO and CSR, both methods of CSR."""
     A = CASES[name]
     owned = A.sorted() if fmt == "coo" else cs.coo2csr(A.copy())
+    if -(-A.width // pw) > 2048:  # more panels than the encoding holds
+        with pytest.raises(cs.ClspmvError) as e:
+            cs.DeviceMatrix(ctx, owned, panel=pw)
+        assert e.value.code == cs.ERR_UNSUPPORTED
+        return
     for method in ([0] if fmt == "coo" else [0, 1]):
         _run(ctx, oracle, A, owned, method, panel=pw)
 
@@ -293,3 +298,63 @@ def test_16bit_needs_panel(ctx):
     for panel in (0, cs.PANEL_BOTH):
         with pytest.raises(cs.ClspmvError):
             cs.DeviceMatrix(ctx, A.sorted(), panel=panel, dtype=cs.DT_BF16)
+
+
+MULTI_CASES = ["rand_small", "rand_dyadic", "rand_long_rows", "rand_long_dyadic", "lap2d_256",
+               "stencil27_dyadic", "banded", "tall", "wide"]
+
+
+@pytest.mark.parametrize("name", MULTI_CASES)
+@pytest.mark.parametrize("fmt", ["dia", "csr"])
+@pytest.mark.parametrize("dtype", [cs.DT_F32, cs.DT_BF16, cs.DT_F16])
+@pytest.mark.parametrize("k", [1, 3, 4, 8, 16])
+def test_spmv_multi(ctx, oracle, name, fmt, dtype, k):
+    """One pass for k right-hand sides (X row-major, Y column-major) equals k
+    oracle SpMVs; 16-bit values are compared with the values as stored."""
+    A = CASES[name]
+    if fmt == "dia" and A.height != A.width:
+        pytest.skip("DIA conversion is defined for square matrices")
+    owned = cs.coo2dia(A) if fmt == "dia" else cs.coo2csr(A.copy())
+    if owned is None:
+        pytest.skip("DIA refused: padded size >= 2*nnz (include/matrix_storage.h:972)")
+    dyadic = _is_dyadic(A.val)
+    rng = np.random.default_rng(k)
+    if dyadic:
+        X = (rng.integers(-64, 65, size=(A.width, k)) / 64.0).astype(np.float32)
+        Y0 = (rng.integers(-64, 65, size=(k, A.height + 3)) / 64.0).astype(np.float32)
+    else:
+        X = rng.uniform(-1, 1, size=(A.width, k)).astype(np.float32)
+        Y0 = rng.uniform(-1, 1, size=(k, A.height + 3)).astype(np.float32)
+    ldy = A.height + 3
+    dA = cs.DeviceMatrix(ctx, owned, panel=0 if fmt == "csr" else None, dtype=dtype)
+    vals = A.val if dtype == cs.DT_F32 else cs.convert_to_f32(cs.convert_from_f32(A.val, dtype), dtype)
+    dX = cs.DeviceVector(ctx, X.size, X.reshape(-1))
+    dY = cs.DeviceVector(ctx, Y0.size, Y0.reshape(-1))
+    dA.spmv_multi(k, dX, dY, ldy)
+    ctx.sync()
+    Y = dY.download().reshape(k, ldy)
+    for q in range(k):
+        y64, bound = oracle.spmv_f64(A.row, A.col, vals, X[:, q].copy(), A.height, Y0[q, :A.height].copy())
+        err, worst = oracle.scaled_error(Y[q, :A.height].copy(), y64, bound)
+        assert err <= TOL, f"rhs {q}: {err:.3e} at row {worst}"
+        if dyadic:
+            np.testing.assert_array_equal(Y[q, :A.height], y64.astype(np.float32))
+        np.testing.assert_array_equal(Y[q, A.height:], Y0[q, A.height:])
+
+
+@pytest.mark.parametrize("name", ["rand_small", "rand_dyadic", "rand_long_rows"])
+@pytest.mark.parametrize("dtype", [cs.DT_BF16, cs.DT_F16])
+def test_csr_16bit_values(ctx, oracle, name, dtype):
+    """CSR's own arrays with 16-bit values through clspmv_spmv (every method)."""
+    A = CASES[name]
+    for method in (0, 1):
+        _run(ctx, oracle, A, cs.coo2csr(A.copy()), method, panel=0, dtype=dtype)
+
+
+def test_spmv_multi_refuses_other_formats(ctx):
+    A = CASES["rand_small"]
+    dA = cs.DeviceMatrix(ctx, cs.coo2ell(A.copy()))
+    dX = cs.DeviceVector(ctx, A.width * 2)
+    dY = cs.DeviceVector(ctx, A.height * 2)
+    with pytest.raises(cs.ClspmvError):
+        dA.spmv_multi(2, dX, dY, A.height)

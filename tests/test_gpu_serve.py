@@ -1,7 +1,8 @@
 """GPU tests of the serving loop (clspmv_serve.h): every request's result is
-the same alone (plain per-request launches, one at a time) and in the mix
-(continuous batching over several streams, graphs replayed), bit for bit,
-and matches the FP64 oracle within 1e-5 of |A||x|."""
+the same alone (submitted and finished one at a time) and in the mix
+(continuous batching over several streams, graphs replayed), bit for bit;
+results and the plain per-request baseline match the FP64 oracle within
+1e-5 of |A||x|."""
 
 import numpy as np
 import pytest
@@ -16,7 +17,8 @@ TOL = 1e-5
 
 @pytest.fixture(scope="module")
 def resident(ctx):
-    """(coo, device matrix, method, dtype) of four matrices of different formats."""
+    """(coo, device matrix, method, dtype) of matrices of different formats
+    and value types."""
     out = []
     lap = matrices.laplacian_2d(64)
     out.append((lap, cs.DeviceMatrix(ctx, cs.coo2dia(lap.copy())), 0, None))
@@ -24,6 +26,10 @@ def resident(ctx):
     out.append((pl, cs.DeviceMatrix(ctx, pl.sorted(), panel=cs.PANEL_AUTO), 0, None))
     rnd = matrices.random_sparse(3000, 2500, 0.004, seed=11, long_rows=3, long_len=900)
     out.append((rnd, cs.DeviceMatrix(ctx, cs.coo2csr(rnd.copy())), 1, None))
+    lap16 = matrices.laplacian_2d(40)
+    out.append((lap16, cs.DeviceMatrix(ctx, cs.coo2dia(lap16.copy()), dtype=cs.DT_BF16), 0, cs.DT_BF16))
+    pl16 = matrices.power_law(9000, seed=7, max_len=300, band=())
+    out.append((pl16, cs.DeviceMatrix(ctx, cs.coo2csr(pl16.copy()), panel=0, dtype=cs.DT_BF16), 0, cs.DT_BF16))
     blk = matrices.block_stencil(500, 4, 4, seed=4, offsets=(0, 1, -1, 9))
     out.append((blk, cs.BlockSpmm(ctx, cs.coo2bcsr(blk.copy(), 4, 4), cs.DT_BF16), 0, cs.DT_BF16))
     assert out[1][1].info.panel_active == 1
@@ -48,11 +54,12 @@ def _requests(resident, ids, n, seed):
 
 
 def _oracle_check(oracle, resident, m, k, seed, y):
-    A, _, _, dtype = resident[m]
+    A, dm, _, dtype = resident[m]
     vals = A.val if dtype is None else cs.convert_to_f32(cs.convert_from_f32(A.val, dtype), dtype)
+    block = isinstance(dm, cs.BlockSpmm)  # x itself is rounded to 16 bits there
     for j in range(k):
         x = cs.request_x(seed, j, A.width)
-        if dtype is not None:
+        if block:
             x = cs.convert_to_f32(cs.convert_from_f32(x, dtype), dtype)
         y64, bound = oracle.spmv_f64(A.row, A.col, vals, x, A.height)
         err, worst = oracle.scaled_error(y[j * A.height:(j + 1) * A.height].copy(), y64, bound)
@@ -60,21 +67,25 @@ def _oracle_check(oracle, resident, m, k, seed, y):
 
 
 def test_alone_equals_mix_and_oracle(ctx, oracle, resident):
+    """Each request alone on the server (submitted and waited for one at a
+    time) and in the mix (all at once, several streams) gives the same
+    result bit for bit; the plain per-request baseline (one SpMV launch per
+    right-hand side) agrees with both within the tolerance."""
     s, ids = _server(resident, streams=3, max_batch=8)
-    alone = _requests(resident, ids, 48, seed=1)
-    mixed = [(m, k, sd, t, np.zeros_like(y)) for (m, k, sd, t, y) in alone]
-    lat1, _ = s.replay(alone, mode=1)
+    mixed = _requests(resident, ids, 48, seed=1)
+    alone = [(m, k, sd, t, np.zeros_like(y)) for (m, k, sd, t, y) in mixed]
+    plain = [(m, k, sd, t, np.zeros_like(y)) for (m, k, sd, t, y) in mixed]
+    for (m, k, sd, _, y) in alone:
+        s.wait(s.submit(m, k, sd, y))
     lat0, _ = s.replay(mixed, mode=0)
+    lat1, _ = s.replay(plain, mode=1)
     s.close()
     assert np.all(lat0 >= 0) and np.all(lat1 >= 0)
     for (m, k, sd, _, ya), (_, _, _, _, ym) in zip(alone, mixed):
         np.testing.assert_array_equal(ya, ym, err_msg=f"matrix {m} seed {sd}")
-    checked = set()
-    for (m, k, sd, _, y) in alone:
-        if m not in checked:
-            _oracle_check(oracle, resident, m, k, sd, y)
-            checked.add(m)
-    assert len(checked) == len(resident)
+    for (m, k, sd, _, y), (_, _, _, _, yp) in zip(mixed, plain):
+        _oracle_check(oracle, resident, m, k, sd, y)
+        _oracle_check(oracle, resident, m, k, sd, yp)
 
 
 @pytest.mark.parametrize("graphs", [False, True])
@@ -83,7 +94,7 @@ def test_graphs_and_budget(ctx, resident, graphs):
     admits one request at a time still serves all of them."""
     ref, ids = _server(resident, streams=2, max_batch=4)
     reqs = _requests(resident, ids, 24, seed=2)
-    ref.replay(reqs, mode=1)
+    ref.replay(reqs, mode=0)
     ref.close()
     biggest = max(4 * 8 * (A.width + A.height + 8) for (A, _, _, _) in resident)
     s, _ = _server(resident, streams=4, max_batch=16, hbm_budget=biggest, use_graphs=graphs)
