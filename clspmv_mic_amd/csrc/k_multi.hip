@@ -4,7 +4,8 @@
 // X is row-major (the k values of column c are X[c*k .. c*k+k)), so the
 // gather of one column brings every right-hand side's value in one or a few
 // 16-byte loads: a random column costs one cache line for all k products
-// instead of one line per product.  Y is column-major (ldy), so each
+// instead of one line per product.  CSR rows longer than kMultiLongRow get
+// a work-group of their own.  Y is column-major (ldy), so each
 // right-hand side's y stays a contiguous vector; lanes that own consecutive
 // rows write consecutive addresses.  Matrix values may be FP32 or 16-bit
 // (widened on load); sums are FP32 in a fixed order, no atomics.
@@ -78,6 +79,7 @@ __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
     const int lane = threadIdx.x & (G - 1);
     if (row >= rows) return;  // a lane group exits together
     const int s = rp[row], e = rp[row + 1];
+    if (e - s > launch::kMultiLongRow) return;  // csr_multi_long_kernel's row
     float acc[KB];
 #pragma unroll
     for (int q = 0; q < KB; ++q) acc[q] = 0.0f;
@@ -95,6 +97,43 @@ __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
 #pragma unroll
         for (int q = 0; q < KB; ++q)
             if (q < k) Y[(size_t)q * ldy + row] += acc[q];
+    }
+}
+
+// Rows longer than kMultiLongRow (power-law tails): one work-group per row,
+// its four waves' partial sums combined in a fixed order through LDS.
+template <int KB, class V>
+__global__ __launch_bounds__(kBlock) void csr_multi_long_kernel(
+    const int* __restrict__ long_rows, const int* __restrict__ rp, const int* __restrict__ col,
+    const typename V::T* __restrict__ val, int k, const float* __restrict__ X, bool v4,
+    float* __restrict__ Y, int ldy) {
+    __shared__ float part[kBlock / kWave][KB];
+    const int row = long_rows[blockIdx.x];
+    const int t = threadIdx.x;
+    const int s = rp[row], e = rp[row + 1];
+    float acc[KB];
+#pragma unroll
+    for (int q = 0; q < KB; ++q) acc[q] = 0.0f;
+    for (int j = s + t; j < e; j += kBlock) {
+        const int c = ld_stream(col + j);
+        const float v = V::ld(val + j);
+        float xv[KB];
+        load_xrow<KB>(X, c, k, v4, xv);
+#pragma unroll
+        for (int q = 0; q < KB; ++q) acc[q] = fmaf(v, xv[q], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < KB; ++q) acc[q] = group_sum<kWave>(acc[q]);
+    if ((t & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int q = 0; q < KB; ++q) part[t / kWave][q] = acc[q];
+    }
+    __syncthreads();
+    if (t < KB && t < k) {
+        float tot = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; ++w) tot += part[w][t];
+        Y[(size_t)t * ldy + row] += tot;
     }
 }
 
@@ -141,13 +180,33 @@ hipError_t csr_multi_kb(hipStream_t s, int rows, int lanes, const int* rp, const
     }
 }
 
+template <int KB, class V>
+hipError_t csr_multi_long_kb(hipStream_t s, int nlong, const int* long_rows, const int* rp, const int* col,
+                             const void* val, int k, const float* X, bool v4, float* Y, int ldy) {
+    if (nlong <= 0) return hipSuccess;
+    hipLaunchKernelGGL((csr_multi_long_kernel<KB, V>), dim3(nlong), dim3(kBlock), 0, s, long_rows, rp, col,
+                       static_cast<const typename V::T*>(val), k, X, v4, Y, ldy);
+    return hipGetLastError();
+}
+
 template <class V>
 hipError_t csr_multi_v(hipStream_t s, int rows, int lanes, const int* rp, const int* col, const void* val,
-                       int k, const float* X, bool v4, float* Y, int ldy) {
-    if (k <= 1) return csr_multi_kb<1, V>(s, rows, lanes, rp, col, val, k, X, false, Y, ldy);
-    if (k <= 4) return csr_multi_kb<4, V>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
-    if (k <= 8) return csr_multi_kb<8, V>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
-    return csr_multi_kb<16, V>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
+                       int nlong, const int* long_rows, int k, const float* X, bool v4, float* Y, int ldy) {
+    hipError_t e;
+    if (k <= 1) {
+        e = csr_multi_kb<1, V>(s, rows, lanes, rp, col, val, k, X, false, Y, ldy);
+        if (e == hipSuccess) e = csr_multi_long_kb<1, V>(s, nlong, long_rows, rp, col, val, k, X, false, Y, ldy);
+    } else if (k <= 4) {
+        e = csr_multi_kb<4, V>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
+        if (e == hipSuccess) e = csr_multi_long_kb<4, V>(s, nlong, long_rows, rp, col, val, k, X, v4, Y, ldy);
+    } else if (k <= 8) {
+        e = csr_multi_kb<8, V>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
+        if (e == hipSuccess) e = csr_multi_long_kb<8, V>(s, nlong, long_rows, rp, col, val, k, X, v4, Y, ldy);
+    } else {
+        e = csr_multi_kb<16, V>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
+        if (e == hipSuccess) e = csr_multi_long_kb<16, V>(s, nlong, long_rows, rp, col, val, k, X, v4, Y, ldy);
+    }
+    return e;
 }
 
 }  // namespace
@@ -168,14 +227,15 @@ hipError_t dia_multi(hipStream_t s, int rows, int cols, int len, int ndiag, cons
 }
 
 hipError_t csr_multi(hipStream_t s, int rows, int lanes, const int* rp, const int* col, int dtype,
-                     const void* val, int k, const float* X, float* Y, int ldy) {
+                     const void* val, int nlong, const int* long_rows, int k, const float* X, float* Y,
+                     int ldy) {
     if (rows <= 0 || k <= 0) return hipSuccess;
     if (k > kMultiMax) return hipErrorInvalidValue;
     const bool v4 = k % 4 == 0 && ((uintptr_t)X & 15) == 0;
     switch (dtype) {
-        case 0: return csr_multi_v<ValF32>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
-        case 1: return csr_multi_v<ValBF16>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
-        case 2: return csr_multi_v<ValF16>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
+        case 0: return csr_multi_v<ValF32>(s, rows, lanes, rp, col, val, nlong, long_rows, k, X, v4, Y, ldy);
+        case 1: return csr_multi_v<ValBF16>(s, rows, lanes, rp, col, val, nlong, long_rows, k, X, v4, Y, ldy);
+        case 2: return csr_multi_v<ValF16>(s, rows, lanes, rp, col, val, nlong, long_rows, k, X, v4, Y, ldy);
         default: return hipErrorInvalidValue;
     }
 }

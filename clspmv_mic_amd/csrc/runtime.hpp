@@ -26,6 +26,8 @@ struct clspmv_dmat {
     float* val = nullptr;
     int* blocks = nullptr;
     int nblocks = 0;
+    int* long_rows = nullptr;  // rows longer than launch::kMultiLongRow
+    int nlong = 0;
     int lanes = 1;
     int max_row = 0;
     // COO: row/col/val, carry slots (2 per chunk).
