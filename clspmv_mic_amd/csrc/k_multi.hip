@@ -4,8 +4,9 @@
 // X is row-major (the k values of column c are X[c*k .. c*k+k)), so the
 // gather of one column brings every right-hand side's value in one or a few
 // 16-byte loads: a random column costs one cache line for all k products
-// instead of one line per product.  CSR rows longer than kMultiLongRow get
-// a work-group of their own.  Y is column-major (ldy), so each
+// instead of one line per product.  CSR runs over the adaptive row blocks
+// (a work-group per block of short rows or per long row).  Y is
+// column-major (ldy), so each
 // right-hand side's y stays a contiguous vector; lanes that own consecutive
 // rows write consecutive addresses.  Matrix values may be FP32 or 16-bit
 // (widened on load); sums are FP32 in a fixed order, no atomics.
@@ -67,73 +68,127 @@ __global__ __launch_bounds__(kBlock) void dia_multi_kernel(
         if (q < k) Y[(size_t)q * ldy + r] += acc[q];
 }
 
-// G lanes per row stride over it; the per-lane partial sums of every
-// right-hand side are then combined by an xor tree over the G lanes.
-template <int G, int KB, class V>
-__global__ __launch_bounds__(kBlock) void csr_multi_kernel(
-    int rows, const int* __restrict__ rp, const int* __restrict__ col,
-    const typename V::T* __restrict__ val, int k, const float* __restrict__ X, bool v4,
-    float* __restrict__ Y, int ldy) {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int row = (tile * kBlock + (int)threadIdx.x) / G;
-    const int lane = threadIdx.x & (G - 1);
-    if (row >= rows) return;  // a lane group exits together
-    const int s = rp[row], e = rp[row + 1];
-    if (e - s > launch::kMultiLongRow) return;  // csr_multi_long_kernel's row
-    float acc[KB];
-#pragma unroll
-    for (int q = 0; q < KB; ++q) acc[q] = 0.0f;
-    for (int j = s + lane; j < e; j += G) {
-        const int c = ld_stream(col + j);
-        const float v = V::ld(val + j);
-        float xv[KB];
-        load_xrow<KB>(X, c, k, v4, xv);
-#pragma unroll
-        for (int q = 0; q < KB; ++q) acc[q] = fmaf(v, xv[q], acc[q]);
+// Loads right-hand sides qb .. qb+Q-1 of row c of X (zero where q >= k).
+template <int Q>
+__device__ __forceinline__ void load_xq(const float* __restrict__ X, int c, int k, int qb, bool vec,
+                                        float* xv) {
+    const float* p = X + (size_t)c * k + qb;
+    if constexpr (Q == 4) {
+        if (vec) {  // k % 4 == 0, X 16-byte aligned
+            const float4 t = *reinterpret_cast<const float4*>(p);
+            xv[0] = t.x; xv[1] = t.y; xv[2] = t.z; xv[3] = t.w;
+            return;
+        }
+    } else if constexpr (Q == 2) {
+        if (vec) {  // k == 2, X 8-byte aligned
+            const float2 t = *reinterpret_cast<const float2*>(p);
+            xv[0] = t.x; xv[1] = t.y;
+            return;
+        }
     }
 #pragma unroll
-    for (int q = 0; q < KB; ++q) acc[q] = group_sum<G>(acc[q]);
-    if (lane == 0) {
+    for (int q = 0; q < Q; ++q) xv[q] = qb + q < k ? p[q] : 0.0f;
+}
+
+// Sum over the lanes of a group of G (a power of two <= 64) lanes that
+// share lane % L: xor steps G/2 .. L.
+__device__ __forceinline__ float quad_sum(float v, int g, int l) {
+    for (int m = g >> 1; m >= l; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+// Accumulates a[q] += val[j] * X[col[j]][qb + q] over j = j0, j0 + step, ...
+// below j1, four nonzeros in flight per lane.
+template <int Q, class V>
+__device__ __forceinline__ void csr_multi_run(const int* __restrict__ col,
+                                              const typename V::T* __restrict__ val, int j0, int j1,
+                                              int step, const float* __restrict__ X, int k, int qb,
+                                              bool vec, float* a) {
+    int j = j0;
+    for (; j + 3 * step < j1; j += 4 * step) {
+        int c[4];
+        float v[4];
 #pragma unroll
-        for (int q = 0; q < KB; ++q)
-            if (q < k) Y[(size_t)q * ldy + row] += acc[q];
+        for (int u = 0; u < 4; ++u) {
+            c[u] = ld_stream(col + j + u * step);
+            v[u] = V::ld(val + j + u * step);
+        }
+        float xv[4][Q];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) load_xq<Q>(X, c[u], k, qb, vec, xv[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) a[q] = fmaf(v[u], xv[u][q], a[q]);
+    }
+    for (; j < j1; j += step) {
+        const int c = ld_stream(col + j);
+        const float v = V::ld(val + j);
+        float xv[Q];
+        load_xq<Q>(X, c, k, qb, vec, xv);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) a[q] = fmaf(v, xv[q], a[q]);
     }
 }
 
-// Rows longer than kMultiLongRow (power-law tails): one work-group per row,
-// its four waves' partial sums combined in a fixed order through LDS.
+// CSR over the adaptive row blocks (runs of rows whose nonzeros fit
+// kAdaptiveNnz, or one longer row).  Each lane carries Q = min(KB, 4)
+// right-hand sides (one 16-byte gather of X per nonzero), and L = KB / Q
+// neighbouring lanes cover the k values of one nonzero.  Short rows: a
+// group of G lanes per row, G the largest power of two with the block's
+// rows fitting the work-group (at least L, at most a wave; rows beyond the
+// work-group loop).  A long row: the whole work-group, the four waves'
+// partial sums combined in a fixed order through LDS.  No product staging,
+// so occupancy is set by registers alone.
 template <int KB, class V>
-__global__ __launch_bounds__(kBlock) void csr_multi_long_kernel(
-    const int* __restrict__ long_rows, const int* __restrict__ rp, const int* __restrict__ col,
-    const typename V::T* __restrict__ val, int k, const float* __restrict__ X, bool v4,
+__global__ __launch_bounds__(kBlock) void csr_multi_kernel(
+    const int* __restrict__ blocks, const int* __restrict__ rp, const int* __restrict__ col,
+    const typename V::T* __restrict__ val, int k, const float* __restrict__ X, bool vec,
     float* __restrict__ Y, int ldy) {
+    constexpr int Q = KB < 4 ? KB : 4;
+    constexpr int L = KB / Q;
     __shared__ float part[kBlock / kWave][KB];
-    const int row = long_rows[blockIdx.x];
+    const int b = blockIdx.x;
+    const int r0 = blocks[b], r1 = blocks[b + 1];
+    const int nrows = r1 - r0;
     const int t = threadIdx.x;
-    const int s = rp[row], e = rp[row + 1];
-    float acc[KB];
+    const int sub = t % L;
+    const int qb = sub * Q;
+    float a[Q];
+    if (nrows == 1 && rp[r1] - rp[r0] > launch::kAdaptiveNnz) {
 #pragma unroll
-    for (int q = 0; q < KB; ++q) acc[q] = 0.0f;
-    for (int j = s + t; j < e; j += kBlock) {
-        const int c = ld_stream(col + j);
-        const float v = V::ld(val + j);
-        float xv[KB];
-        load_xrow<KB>(X, c, k, v4, xv);
+        for (int q = 0; q < Q; ++q) a[q] = 0.0f;
+        csr_multi_run<Q, V>(col, val, rp[r0] + t / L, rp[r1], kBlock / L, X, k, qb, vec, a);
 #pragma unroll
-        for (int q = 0; q < KB; ++q) acc[q] = fmaf(v, xv[q], acc[q]);
+        for (int q = 0; q < Q; ++q) a[q] = quad_sum(a[q], kWave, L);
+        if ((t & (kWave - 1)) < L) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) part[t / kWave][qb + q] = a[q];
+        }
+        __syncthreads();
+        if (t < KB && t < k) {
+            float tot = 0.0f;
+#pragma unroll
+            for (int w = 0; w < kBlock / kWave; ++w) tot += part[w][t];
+            Y[(size_t)t * ldy + r0] += tot;
+        }
+        return;
     }
+    int g = kBlock / (nrows > 0 ? nrows : 1);
+    g = g >= 64 ? 64 : g >= 32 ? 32 : g >= 16 ? 16 : g >= 8 ? 8 : g >= 4 ? 4 : g >= 2 ? 2 : 1;
+    if (g < L) g = L;
+    const int lane = t & (g - 1);
+    for (int r = t / g; r < nrows; r += kBlock / g) {  // uniform per group
 #pragma unroll
-    for (int q = 0; q < KB; ++q) acc[q] = group_sum<kWave>(acc[q]);
-    if ((t & (kWave - 1)) == 0) {
+        for (int q = 0; q < Q; ++q) a[q] = 0.0f;
+        csr_multi_run<Q, V>(col, val, rp[r0 + r] + lane / L, rp[r0 + r + 1], g / L, X, k, qb, vec, a);
 #pragma unroll
-        for (int q = 0; q < KB; ++q) part[t / kWave][q] = acc[q];
-    }
-    __syncthreads();
-    if (t < KB && t < k) {
-        float tot = 0.0f;
+        for (int q = 0; q < Q; ++q) a[q] = quad_sum(a[q], g, L);
+        if (lane < L) {
 #pragma unroll
-        for (int w = 0; w < kBlock / kWave; ++w) tot += part[w][t];
-        Y[(size_t)t * ldy + row] += tot;
+            for (int q = 0; q < Q; ++q)
+                if (qb + q < k) Y[(size_t)(qb + q) * ldy + r0 + r] += a[q];
+        }
     }
 }
 
@@ -156,57 +211,24 @@ hipError_t dia_multi_v(hipStream_t s, int rows, int cols, int len, int ndiag, co
     return dia_multi_kb<16, V>(s, rows, cols, len, ndiag, off, data, k, X, v4, Y, ldy);
 }
 
-template <int G, int KB, class V>
-hipError_t csr_multi_g(hipStream_t s, int rows, const int* rp, const int* col, const void* val, int k,
-                       const float* X, bool v4, float* Y, int ldy) {
-    hipLaunchKernelGGL((csr_multi_kernel<G, KB, V>), dim3(ceil_div((long long)rows * G, kBlock)),
-                       dim3(kBlock), 0, s, rows, rp, col, static_cast<const typename V::T*>(val), k, X, v4,
-                       Y, ldy);
-    return hipGetLastError();
-}
-
 template <int KB, class V>
-hipError_t csr_multi_kb(hipStream_t s, int rows, int lanes, const int* rp, const int* col, const void* val,
-                        int k, const float* X, bool v4, float* Y, int ldy) {
-    switch (lanes) {
-        case 1: return csr_multi_g<1, KB, V>(s, rows, rp, col, val, k, X, v4, Y, ldy);
-        case 2: return csr_multi_g<2, KB, V>(s, rows, rp, col, val, k, X, v4, Y, ldy);
-        case 4: return csr_multi_g<4, KB, V>(s, rows, rp, col, val, k, X, v4, Y, ldy);
-        case 8: return csr_multi_g<8, KB, V>(s, rows, rp, col, val, k, X, v4, Y, ldy);
-        case 16: return csr_multi_g<16, KB, V>(s, rows, rp, col, val, k, X, v4, Y, ldy);
-        case 32: return csr_multi_g<32, KB, V>(s, rows, rp, col, val, k, X, v4, Y, ldy);
-        case 64: return csr_multi_g<64, KB, V>(s, rows, rp, col, val, k, X, v4, Y, ldy);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-template <int KB, class V>
-hipError_t csr_multi_long_kb(hipStream_t s, int nlong, const int* long_rows, const int* rp, const int* col,
-                             const void* val, int k, const float* X, bool v4, float* Y, int ldy) {
-    if (nlong <= 0) return hipSuccess;
-    hipLaunchKernelGGL((csr_multi_long_kernel<KB, V>), dim3(nlong), dim3(kBlock), 0, s, long_rows, rp, col,
-                       static_cast<const typename V::T*>(val), k, X, v4, Y, ldy);
+hipError_t csr_multi_kb(hipStream_t s, int nblocks, const int* blocks, const int* rp, const int* col,
+                        const void* val, int k, const float* X, float* Y, int ldy) {
+    constexpr int Q = KB < 4 ? KB : 4;
+    const bool vec = Q > 1 && k % Q == 0 && ((uintptr_t)X % (Q * sizeof(float))) == 0;
+    hipLaunchKernelGGL((csr_multi_kernel<KB, V>), dim3(nblocks), dim3(kBlock), 0, s, blocks, rp, col,
+                       static_cast<const typename V::T*>(val), k, X, vec, Y, ldy);
     return hipGetLastError();
 }
 
 template <class V>
-hipError_t csr_multi_v(hipStream_t s, int rows, int lanes, const int* rp, const int* col, const void* val,
-                       int nlong, const int* long_rows, int k, const float* X, bool v4, float* Y, int ldy) {
-    hipError_t e;
-    if (k <= 1) {
-        e = csr_multi_kb<1, V>(s, rows, lanes, rp, col, val, k, X, false, Y, ldy);
-        if (e == hipSuccess) e = csr_multi_long_kb<1, V>(s, nlong, long_rows, rp, col, val, k, X, false, Y, ldy);
-    } else if (k <= 4) {
-        e = csr_multi_kb<4, V>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
-        if (e == hipSuccess) e = csr_multi_long_kb<4, V>(s, nlong, long_rows, rp, col, val, k, X, v4, Y, ldy);
-    } else if (k <= 8) {
-        e = csr_multi_kb<8, V>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
-        if (e == hipSuccess) e = csr_multi_long_kb<8, V>(s, nlong, long_rows, rp, col, val, k, X, v4, Y, ldy);
-    } else {
-        e = csr_multi_kb<16, V>(s, rows, lanes, rp, col, val, k, X, v4, Y, ldy);
-        if (e == hipSuccess) e = csr_multi_long_kb<16, V>(s, nlong, long_rows, rp, col, val, k, X, v4, Y, ldy);
-    }
-    return e;
+hipError_t csr_multi_v(hipStream_t s, int nblocks, const int* blocks, const int* rp, const int* col,
+                       const void* val, int k, const float* X, float* Y, int ldy) {
+    if (k <= 1) return csr_multi_kb<1, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+    if (k <= 2) return csr_multi_kb<2, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+    if (k <= 4) return csr_multi_kb<4, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+    if (k <= 8) return csr_multi_kb<8, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+    return csr_multi_kb<16, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
 }
 
 }  // namespace
@@ -226,16 +248,14 @@ hipError_t dia_multi(hipStream_t s, int rows, int cols, int len, int ndiag, cons
     }
 }
 
-hipError_t csr_multi(hipStream_t s, int rows, int lanes, const int* rp, const int* col, int dtype,
-                     const void* val, int nlong, const int* long_rows, int k, const float* X, float* Y,
-                     int ldy) {
-    if (rows <= 0 || k <= 0) return hipSuccess;
+hipError_t csr_multi(hipStream_t s, int nblocks, const int* blocks, const int* rp, const int* col,
+                     int dtype, const void* val, int k, const float* X, float* Y, int ldy) {
+    if (nblocks <= 0 || k <= 0) return hipSuccess;
     if (k > kMultiMax) return hipErrorInvalidValue;
-    const bool v4 = k % 4 == 0 && ((uintptr_t)X & 15) == 0;
     switch (dtype) {
-        case 0: return csr_multi_v<ValF32>(s, rows, lanes, rp, col, val, nlong, long_rows, k, X, v4, Y, ldy);
-        case 1: return csr_multi_v<ValBF16>(s, rows, lanes, rp, col, val, nlong, long_rows, k, X, v4, Y, ldy);
-        case 2: return csr_multi_v<ValF16>(s, rows, lanes, rp, col, val, nlong, long_rows, k, X, v4, Y, ldy);
+        case 0: return csr_multi_v<ValF32>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+        case 1: return csr_multi_v<ValBF16>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+        case 2: return csr_multi_v<ValF16>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
         default: return hipErrorInvalidValue;
     }
 }

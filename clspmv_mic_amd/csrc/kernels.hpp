@@ -100,12 +100,11 @@ hipError_t sbell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
 // ---- multi-right-hand-side SpMV (k <= kMultiMax) --------------------------------
 // Y[:, q] += A X[:, q]; X row-major (X[c*k + q]), Y column-major (ldy).
 constexpr int kMultiMax = 16;
-constexpr int kMultiLongRow = 1024;  // longer CSR rows: one work-group each (long_rows)
 hipError_t dia_multi(hipStream_t s, int rows, int cols, int len, int ndiag, const int* offsets,
                      int dtype, const void* data, int k, const float* X, float* Y, int ldy);
-hipError_t csr_multi(hipStream_t s, int rows, int lanes, const int* rp, const int* col, int dtype,
-                     const void* val, int nlong, const int* long_rows, int k, const float* X, float* Y,
-                     int ldy);
+// CSR over the adaptive row blocks (csr_adaptive's `blocks`).
+hipError_t csr_multi(hipStream_t s, int nblocks, const int* blocks, const int* rp, const int* col,
+                     int dtype, const void* val, int k, const float* X, float* Y, int ldy);
 
 // ---- mixed-precision 4x4-block SpMM (MFMA) -------------------------------------
 // Y[:, 0:k] += A X[:, 0:k]; A: block rows rp/bcol (element columns, multiples

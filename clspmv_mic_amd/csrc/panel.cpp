@@ -264,7 +264,6 @@ void drop_encoding(clspmv_dmat* d, bool panel) {
     release_array(d, &d->carry_row);
     release_array(d, &d->carry_val);
     release_array(d, &d->blocks);
-    release_array(d, &d->long_rows);
     d->plain = false;
     d->use_panel = true;
 }

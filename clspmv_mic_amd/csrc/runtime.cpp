@@ -120,8 +120,8 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
     switch (I.format) {
         case CLSPMV_FMT_CSR:
             if (A->dtype) {  // 16-bit values: the multi-vector kernel with k = 1
-                e = launch::csr_multi(s, I.rows, A->lanes, A->rp, A->col, A->dtype, A->lpval, A->nlong,
-                                      A->long_rows, 1, x, y, I.rows);
+                e = launch::csr_multi(s, A->nblocks, A->blocks, A->rp, A->col, A->dtype, A->lpval, 1, x, y,
+                                      I.rows);
                 break;
             }
             e = method == 0 ? launch::csr_vector(s, I.rows, A->lanes, A->rp, A->col, A->val, x, y)
@@ -198,9 +198,8 @@ int multi_launch(hipStream_t s, const clspmv_dmat* A, int k, const float* X, flo
         e = launch::dia_multi(s, std::min(I.rows, A->len), I.cols, A->len, A->ndiag, A->offs, A->dtype,
                               A->dtype ? (const void*)A->lpval : (const void*)A->val, k, X, Y, ldy);
     } else if (I.format == CLSPMV_FMT_CSR && A->plain) {
-        e = launch::csr_multi(s, I.rows, A->lanes, A->rp, A->col, A->dtype,
-                              A->dtype ? (const void*)A->lpval : (const void*)A->val, A->nlong, A->long_rows,
-                              k, X, Y, ldy);
+        e = launch::csr_multi(s, A->nblocks, A->blocks, A->rp, A->col, A->dtype,
+                              A->dtype ? (const void*)A->lpval : (const void*)A->val, k, X, Y, ldy);
     } else {
         return fail(CLSPMV_ERR_UNSUPPORTED, "multi-vector SpMV exists for DIA and CSR (own arrays) only");
     }
@@ -427,13 +426,7 @@ int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m,
     std::vector<int> blocks = adaptive_blocks(m->csr_row_ptr, rows);
     d->nblocks = (int)blocks.size() - 1;
     UPLOAD_CHECK(upload_array(d, blocks.data(), blocks.size(), &d->blocks));
-    {
-        std::vector<int> longr;
-        for (int r = 0; r < rows; ++r)
-            if (m->csr_row_ptr[r + 1] - m->csr_row_ptr[r] > launch::kMultiLongRow) longr.push_back(r);
-        d->nlong = (int)longr.size();
-        UPLOAD_CHECK(upload_array(d, longr.data(), longr.size(), &d->long_rows));
-    }
+
     if (o.panel_width != 0 && nnz > 0) {
         std::vector<int> row_of(nnz);
         for (int r = 0; r < rows; ++r)
@@ -829,6 +822,7 @@ int clspmv_dmat_set_panel(clspmv_dmat* m, int use_panel) {
     if (use_panel && !m->pw) return fail(CLSPMV_ERR_UNSUPPORTED, "matrix has no panel encoding");
     if (!use_panel && !m->plain) return fail(CLSPMV_ERR_UNSUPPORTED, "matrix has only the panel encoding");
     m->use_panel = use_panel != 0;
+    m->multi_pick.clear();
     return CLSPMV_OK;
 }
 

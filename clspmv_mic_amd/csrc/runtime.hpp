@@ -26,8 +26,6 @@ struct clspmv_dmat {
     float* val = nullptr;
     int* blocks = nullptr;
     int nblocks = 0;
-    int* long_rows = nullptr;  // rows longer than launch::kMultiLongRow
-    int nlong = 0;
     int lanes = 1;
     int max_row = 0;
     // COO: row/col/val, carry slots (2 per chunk).
@@ -59,6 +57,10 @@ struct clspmv_dmat {
     int* pblk_seg = nullptr;   // first segment of each row block (+ end)
     int* prow_slot = nullptr;  // per block: first slot of each row (+ end)
     bool use_panel = false;
+    // Serving: per k, 1 when one multi-vector pass measured faster than k
+    // SpMVs, 0 when not, -1 not measured yet.  Kept with the matrix so that
+    // every server makes the same choice; cleared when the encoding changes.
+    mutable std::vector<signed char> multi_pick;
     bool plain = true;
     // Values stored in 16 bits (dtype CLSPMV_DT_BF16 / CLSPMV_DT_F16): DIA
     // data, and BSPMM blocks (16 values per 4x4 block, row-major; rp / col

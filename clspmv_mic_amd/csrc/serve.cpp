@@ -7,7 +7,8 @@
 // them -- and (3) launches each admitted request on its matrix's stream:
 // generate x from the seed, then replay the captured (matrix, k) graph
 // (zero y, then one multi-vector pass, one SpMM, or k SpMVs).  Callers only
-// touch the queue.
+// touch the queue.  Whether a k runs as one multi-vector pass or as k SpMVs
+// is measured once per device matrix when it is registered.
 
 #include <algorithm>
 #include <chrono>
@@ -96,7 +97,8 @@ struct SMatrix {
     const clspmv_dmat* A = nullptr;
     int method = 0, rows = 0, cols = 0, kmax = 0, ldx = 0, ldy = 0, stream = 0;
     bool lowp = false;
-    bool multi = false;  // k > 1 runs one multi-vector pass (X row-major)
+    bool multi = false;       // the matrix has a multi-vector kernel
+    std::vector<char> pick;   // per k: run one multi-vector pass (X row-major)
     void* X = nullptr;
     float* Y = nullptr;
     float* pinned = nullptr;  // rows * kmax staging for host copies
@@ -142,6 +144,8 @@ size_t request_bytes(const SMatrix& M, int k) {
     return (size_t)k * ((size_t)M.ldx * (M.lowp ? 2 : 4) + (size_t)M.ldy * 4);
 }
 
+bool use_multi(const SMatrix& M, int k) { return M.multi && k > 1 && M.pick[k]; }
+
 // Zero y, then the k products (one multi-vector pass when `multi`, else one
 // SpMV per right-hand side).  Runs on `s` (captured or not).
 int enqueue_pass(hipStream_t s, const SMatrix& M, int k, bool multi) {
@@ -154,7 +158,7 @@ int enqueue_pass(hipStream_t s, const SMatrix& M, int k, bool multi) {
                                   static_cast<const unsigned short*>(M.X), M.ldx, M.Y, M.ldy));
         return CLSPMV_OK;
     }
-    if (multi && M.multi && k > 1) return multi_launch(s, M.A, k, static_cast<const float*>(M.X), M.Y, M.ldy);
+    if (multi) return multi_launch(s, M.A, k, static_cast<const float*>(M.X), M.Y, M.ldy);
     for (int j = 0; j < k; ++j) {
         int st = dmat_launch(s, M.A, M.method, static_cast<const float*>(M.X) + (size_t)j * M.ldx,
                              M.Y + (size_t)j * M.ldy);
@@ -168,7 +172,7 @@ int enqueue_x(hipStream_t s, const SMatrix& M, int k, unsigned long long seed, b
     if (M.lowp)
         hipLaunchKernelGGL(fill_x16_kernel, dim3(grid_for(n)), dim3(256), 0, s, seed, k, M.cols, M.ldx,
                            M.A->dtype, static_cast<unsigned short*>(M.X));
-    else if (multi && M.multi && k > 1)
+    else if (multi)
         hipLaunchKernelGGL(fill_x_rows_kernel, dim3(grid_for(n)), dim3(256), 0, s, seed, k, M.cols,
                            static_cast<float*>(M.X));
     else
@@ -182,7 +186,7 @@ int graph_for(clspmv_server* S, SMatrix& M, int k, hipGraphExec_t* out) {
     if (!M.exec[k]) {
         hipStream_t s = S->streams[M.stream];
         CLSPMV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int st = enqueue_pass(s, M, k, true);
+        int st = enqueue_pass(s, M, k, use_multi(M, k));
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s, &g);
         if (st) {
@@ -197,6 +201,59 @@ int graph_for(clspmv_server* S, SMatrix& M, int k, hipGraphExec_t* out) {
     return CLSPMV_OK;
 }
 
+// Mean time of `reps` runs of `fn` on `s` after one untimed run, in ms.
+template <class F>
+int time_pass(hipStream_t s, hipEvent_t e0, hipEvent_t e1, F fn, float* ms) {
+    const int reps = 3;
+    int st = fn();
+    if (st) return st;
+    CLSPMV_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps && !st; ++i) st = fn();
+    if (st) return st;
+    CLSPMV_HIP(hipEventRecord(e1, s));
+    CLSPMV_HIP(hipEventSynchronize(e1));
+    CLSPMV_HIP(hipEventElapsedTime(ms, e0, e1));
+    *ms /= reps;
+    return CLSPMV_OK;
+}
+
+// Fills M.pick: for each k in 2..kmax, one multi-vector pass where it
+// measured faster than k SpMVs of the chosen method.  Measured on `s` (a
+// stream of its own, at registration, under the server lock) once per
+// device matrix -- the choice is kept on it -- so servers sharing a matrix
+// run the same kernels and give the same results.
+int calibrate(SMatrix& M, hipStream_t s) {
+    M.pick.assign(M.kmax + 1, 0);
+    if (!M.multi) return CLSPMV_OK;
+    std::vector<signed char>& cache = M.A->multi_pick;
+    if ((int)cache.size() < M.kmax + 1) cache.resize(M.kmax + 1, -1);
+    bool need = false;
+    for (int k = 2; k <= M.kmax; ++k) need = need || cache[k] < 0;
+    if (need) {
+        const long long n = (long long)M.ldx * M.kmax;
+        hipLaunchKernelGGL(fill_x_kernel, dim3(grid_for(n)), dim3(256), 0, s, 12345ull, M.kmax, M.cols,
+                           M.ldx, static_cast<float*>(M.X));
+        CLSPMV_HIP(hipGetLastError());
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        CLSPMV_HIP(hipEventCreate(&e0));
+        int st = CLSPMV_OK;
+        if (hipEventCreate(&e1) != hipSuccess) st = fail(CLSPMV_ERR_HIP, "hipEventCreate failed");
+        float t1 = 0.0f;
+        if (!st) st = time_pass(s, e0, e1, [&] { return enqueue_pass(s, M, 1, false); }, &t1);
+        for (int k = 2; k <= M.kmax && !st; ++k) {
+            if (cache[k] >= 0) continue;
+            float tm = 0.0f;
+            st = time_pass(s, e0, e1, [&] { return enqueue_pass(s, M, k, true); }, &tm);
+            if (!st) cache[k] = tm < k * t1 ? 1 : 0;
+        }
+        (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) return st;
+    }
+    for (int k = 2; k <= M.kmax; ++k) M.pick[k] = cache[k] == 1;
+    return CLSPMV_OK;
+}
+
 int copy_out(hipStream_t s, const SMatrix& M, int k, float* dst) {
     CLSPMV_HIP(hipMemcpy2DAsync(dst, sizeof(float) * M.rows, M.Y, sizeof(float) * M.ldy,
                                 sizeof(float) * M.rows, k, hipMemcpyDeviceToHost, s));
@@ -207,13 +264,13 @@ int copy_out(hipStream_t s, const SMatrix& M, int k, float* dst) {
 int launch_request(clspmv_server* S, SRequest& q) {
     SMatrix& M = S->mats[q.r.matrix];
     hipStream_t s = S->streams[M.stream];
-    int st = enqueue_x(s, M, q.r.k, q.r.seed, true);
+    int st = enqueue_x(s, M, q.r.k, q.r.seed, use_multi(M, q.r.k));
     if (st) return st;
     if (S->cfg.use_graphs) {
         hipGraphExec_t ex = nullptr;
         if ((st = graph_for(S, M, q.r.k, &ex))) return st;
         CLSPMV_HIP(hipGraphLaunch(ex, s));
-    } else if ((st = enqueue_pass(s, M, q.r.k, true))) {
+    } else if ((st = enqueue_pass(s, M, q.r.k, use_multi(M, q.r.k)))) {
         return st;
     }
     if (q.r.host_y && (st = copy_out(s, M, q.r.k, M.pinned))) return st;
@@ -409,6 +466,16 @@ int clspmv_server_add_matrix(clspmv_server* S, const clspmv_dmat* A, int method,
         return fail(CLSPMV_ERR_ALLOC, "request buffers: %s", hipGetErrorString(e));
     }
     M.stream = (int)(S->mats.size() % S->streams.size());
+    hipStream_t cs = nullptr;
+    int st = hipStreamCreate(&cs) == hipSuccess ? CLSPMV_OK : fail(CLSPMV_ERR_HIP, "hipStreamCreate failed");
+    if (!st) st = calibrate(M, cs);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (st) {
+        (void)hipFree(M.X);
+        (void)hipFree(M.Y);
+        (void)hipHostFree(M.pinned);
+        return st;
+    }
     *id = (int)S->mats.size();
     S->mats.push_back(std::move(M));
     return CLSPMV_OK;

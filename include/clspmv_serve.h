@@ -50,9 +50,12 @@ int clspmv_server_destroy(clspmv_server* s);
  * server) and allocates its request buffers for up to kmax right-hand
  * sides.  FP32 matrices run `method` (clspmv_spmv numbering) once per
  * right-hand side; CLSPMV_FMT_BSPMM matrices run clspmv_spmm with x rounded
- * to the matrix's 16-bit type.  DIA and CSR matrices run requests with
- * k > 1 as one multi-vector pass (clspmv_spmv_multi) instead.  *id receives
- * the matrix id. */
+ * to the matrix's 16-bit type.  DIA and CSR matrices (CSR with its plain
+ * encoding) run a request with k > 1 as one multi-vector pass
+ * (clspmv_spmv_multi) instead wherever that was faster: the first
+ * registration of a device matrix times both for each k up to kmax, and the
+ * choice stays with the matrix (every server then runs the same kernels;
+ * clspmv_dmat_set_panel clears it).  *id receives the matrix id. */
 int clspmv_server_add_matrix(clspmv_server* s, const clspmv_dmat* A, int method, int kmax,
                              int* id);
 
