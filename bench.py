@@ -122,7 +122,6 @@ def run_spmm(args):
     import numpy as np
     import clspmv_mic_amd as cs
     from clspmv_mic_amd import matrices
-    import oracle
     dtype = {"bf16": cs.DT_BF16, "fp16": cs.DT_F16}[args.dtype]
     t0 = time.time()
     A = matrices.block_stencil(524_288, 4, 4, seed=4)
@@ -134,7 +133,11 @@ def run_spmm(args):
     print(f"[bench] spmm matrix {A.height}x{A.width} nnz={A.nnz} blocks={blocks} ({time.time() - t0:.1f}s)",
           file=sys.stderr, flush=True)
     vals_s = cs.convert_to_f32(cs.convert_from_f32(A.val, dtype), dtype)
-    orc = oracle.Oracle()
+    # FP32 CSR references for the FP64 verification SpMV on the GPU: the
+    # values before rounding and the values as stored.
+    ref_in = cs.DeviceMatrix(ctx, cs.coo2csr(A.copy()), panel=0)
+    ref_st = cs.DeviceMatrix(ctx, cs.coo2csr(cs.Coo(A.row, A.col, vals_s, A.height, A.width)), panel=0)
+    matrix_error = M.error_report(ref_in, seed=1)
     for k in args.k:
         ldx, ldy = A.width, A.height
         rng = np.random.default_rng(5)
@@ -142,10 +145,17 @@ def run_spmm(args):
         Xb = cs.convert_from_f32(X, dtype)
         Xs = cs.convert_to_f32(Xb, dtype)
         dX = cs.DeviceBuffer(ctx, Xb)
-        dY = cs.DeviceBuffer(ctx, np.zeros((k, ldy), np.float32))
+        dY = cs.DeviceVector(ctx, k * ldy, np.zeros(k * ldy, np.float32))
         M.spmm(k, dX, ldx, dY, ldy)
         ctx.sync()
-        Y1 = dY.download()
+        # Error per right-hand side against the FP64 SpMV on the GPU: of the
+        # stored (rounded) values and x -- the kernel's own error -- and of
+        # the FP32 inputs before rounding -- what 16-bit storage costs.
+        errs, qerrs = [], []
+        for j in range(min(k, args.err_columns)):
+            yj = dY.view(j * ldy)
+            errs.append(cs.verify_spmv(ctx, ref_st, cs.DeviceVector(ctx, ldx, Xs[j]), yj)["max_scaled_error"])
+            qerrs.append(cs.verify_spmv(ctx, ref_in, cs.DeviceVector(ctx, ldx, X[j]), yj)["max_scaled_error"])
         for _ in range(args.warmup):
             M.spmm(k, dX, ldx, dY, ldy)
         ctx.sync()
@@ -156,15 +166,6 @@ def run_spmm(args):
         ms = (time.perf_counter() - t1) * 1e3 / args.steps
         flops = 2.0 * A.nnz * k
         nbytes = spmm_bytes(blocks, brows, A.height, A.width, k)
-        # Error per right-hand side against the FP64 oracle: of the stored
-        # (rounded) values -- the kernel's own error -- and of the FP32
-        # inputs before rounding -- what 16-bit storage costs.
-        errs, qerrs = [], []
-        for j in range(min(k, args.err_columns)):
-            y64, bound = orc.spmv_f64(A.row, A.col, vals_s, Xs[j], A.height)
-            errs.append(orc.scaled_error(Y1[j], y64, bound)[0])
-            y64q, boundq = orc.spmv_f64(A.row, A.col, A.val, X[j], A.height)
-            qerrs.append(orc.scaled_error(Y1[j], y64q, boundq)[0])
         out = {
             "metric": "block SpMM GFLOP/s (BF16/FP16 values, FP32 accumulation on MFMA)",
             "value": round(flops / (ms * 1e-3) / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1,
@@ -179,6 +180,8 @@ def run_spmm(args):
             "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
             "error_vs_fp64_stored_values": [float(f"{e:.3e}") for e in errs],
             "error_vs_fp64_fp32_inputs": [float(f"{e:.3e}") for e in qerrs],
+            "matrix_error_vs_fp64": {kk: (float(f"{v:.3e}") if isinstance(v, float) else v)
+                                     for kk, v in matrix_error.items()},
             "cpu_baseline": None,
         }
         print(json.dumps(out), flush=True)
@@ -197,7 +200,7 @@ def run_serve(args):
     sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e6), size=nmat)).astype(np.int64)
     ctx = cs.Context(0)
     server = cs.Server(streams=args.streams, max_batch=64, hbm_budget=16 << 30)
-    dev, ids, kinds = [], [], []
+    dev, ids, kinds, errs16 = [], [], [], []
     t0 = time.time()
     for i in range(nmat):
         m = int(sizes[i])
@@ -217,6 +220,10 @@ def run_serve(args):
             csr = cs.coo2csr(A.copy())
             d = cs.DeviceMatrix(ctx, csr, panel=0 if bf16 else cs.PANEL_BOTH, dtype=dt)
             kind = "powerlaw"
+        if bf16:  # per-matrix error of the 16-bit storage against FP64 on the GPU
+            ref = cs.DeviceMatrix(ctx, csr if kind == "powerlaw" else cs.coo2csr(A.copy()), panel=0)
+            errs16.append(d.error_report(ref, seed=100 + i)["max_scaled_error"])
+            del ref
         dev.append(d)
         ids.append(server.add(d, 0, kmax=16))
         kinds.append((kind, "bf16" if bf16 else "f32", A.height, A.nnz))
@@ -260,6 +267,9 @@ def run_serve(args):
         "speedup_vs_plain": round(cap / plain, 2),
         "plain_latency_ms_p50": round(float(np.percentile(latb, 50)) * 1e3, 3),
         "gflops_served": round(flops / wall / 1e9, 2),
+        "bf16_matrix_error_vs_fp64": {"matrices": len(errs16),
+                                      "max": float(f"{max(errs16):.3e}") if errs16 else None,
+                                      "median": float(f"{float(np.median(errs16)):.3e}") if errs16 else None},
         "roofline": None,
         "cpu_baseline": None,
     }

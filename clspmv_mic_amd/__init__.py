@@ -172,6 +172,16 @@ class EvalReport(C.Structure):
                 ("max_diff", C.c_double), ("max_ratio", C.c_double)]
 
 
+class ErrorReport(C.Structure):
+    """clspmv_error_report (include/clspmv_device.h)."""
+    _fields_ = [("max_scaled_error", C.c_double), ("rms_scaled_error", C.c_double),
+                ("max_abs_error", C.c_double), ("worst_row", C.c_int), ("rows", C.c_int),
+                ("rows_over_tol", C.c_longlong), ("tol", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # ---------------------------------------------------------------------------
 # signatures
 
@@ -243,6 +253,8 @@ _sig("clspmv_upload_dia_ex", C.c_int, VP, P(DiaMatrix), P(UploadOpts), P(VP))
 _sig("clspmv_dmat_set_panel", C.c_int, VP, C.c_int)
 _sig("clspmv_upload_bcsr_lowp", C.c_int, VP, P(BcsrMatrix), C.c_int, P(VP))
 _sig("clspmv_spmv_multi", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int)
+_sig("clspmv_verify_spmv", C.c_int, VP, VP, VP, VP, VP, C.c_double, P(ErrorReport))
+_sig("clspmv_dmat_error_report", C.c_int, VP, VP, VP, C.c_ulonglong, C.c_double, P(ErrorReport))
 _sig("clspmv_spmm", C.c_int, VP, VP, C.c_int, VP, C.c_int, VP, C.c_int)
 _sig("clspmv_convert_from_f32", C.c_int, PF, VP, C.c_size_t, C.c_int)
 _sig("clspmv_convert_to_f32", C.c_int, VP, PF, C.c_size_t, C.c_int)
@@ -720,6 +732,11 @@ class BlockSpmm:
     def spmm(self, k: int, X, ldx: int, Y, ldy: int):
         check(lib.clspmv_spmm(self.ctx.h, self.h, k, X.p, ldx, Y.p, ldy), "spmm")
 
+    def error_report(self, ref: "DeviceMatrix", seed: int = 1, tol: float = 1e-5) -> dict:
+        """Error of one SpMM column (x rounded to the matrix's type) against
+        the FP64 SpMV of the FP32 CSR `ref` (clspmv_dmat_error_report)."""
+        return error_report(self, ref, seed, tol)
+
     def __del__(self):
         try:
             if self.h:
@@ -765,6 +782,11 @@ class DeviceMatrix:
         check(lib.clspmv_dmat_set_panel(self.h, int(use)), "dmat_set_panel")
         self._refresh()
 
+    def error_report(self, ref: "DeviceMatrix", seed: int = 1, tol: float = 1e-5) -> dict:
+        """Row-scaled error of this matrix's SpMV against the FP64 SpMV of the
+        FP32 CSR `ref` on x = request_x(seed, 0) (clspmv_dmat_error_report)."""
+        return error_report(self, ref, seed, tol)
+
     @property
     def format(self) -> int:
         return self.info.format
@@ -788,6 +810,24 @@ class DeviceMatrix:
                 lib.clspmv_dmat_free(self.h)
         except Exception:
             pass
+
+
+def error_report(A, ref: "DeviceMatrix", seed: int = 1, tol: float = 1e-5) -> dict:
+    """clspmv_dmat_error_report: A (DeviceMatrix or BlockSpmm) against the
+    FP64 SpMV of the FP32 CSR matrix `ref` on the GPU."""
+    rep = ErrorReport()
+    check(lib.clspmv_dmat_error_report(A.ctx.h, A.h, ref.h, seed, tol, C.byref(rep)), "dmat_error_report")
+    return rep.as_dict()
+
+
+def verify_spmv(ctx: Context, ref: "DeviceMatrix", x: DeviceVector, y: DeviceVector,
+                y0: Optional[DeviceVector] = None, tol: float = 1e-5) -> dict:
+    """clspmv_verify_spmv: scores y (= y0 + A x, computed by the caller)
+    against the FP64 SpMV of the FP32 CSR matrix `ref` on the GPU."""
+    rep = ErrorReport()
+    check(lib.clspmv_verify_spmv(ctx.h, ref.h, x.p, y0.p if y0 is not None else None, y.p, tol,
+                                 C.byref(rep)), "verify_spmv")
+    return rep.as_dict()
 
 
 def run_driver(name: str, coo: Coo, ntimes: int = 10, device: int = 0, want_y: bool = True):

@@ -6,10 +6,10 @@
 // 16-byte loads: a random column costs one cache line for all k products
 // instead of one line per product.  CSR runs over the adaptive row blocks
 // (a work-group per block of short rows or per long row).  Y is
-// column-major (ldy), so each
-// right-hand side's y stays a contiguous vector; lanes that own consecutive
-// rows write consecutive addresses.  Matrix values may be FP32 or 16-bit
-// (widened on load); sums are FP32 in a fixed order, no atomics.
+// column-major (ldy), so each right-hand side's y stays a contiguous
+// vector.  Matrix values may be FP32 or 16-bit (widened on load); sums are
+// FP32 in a fixed order, FP64 for CSR rows longer than kLongRowF64; no
+// atomics.
 
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -19,60 +19,16 @@ namespace {
 
 using namespace dev;
 
-// Loads the k (<= KB) values of row c of X.
-template <int KB>
-__device__ __forceinline__ void load_xrow(const float* __restrict__ X, int c, int k, bool v4,
-                                          float* xv) {
-    const float* p = X + (size_t)c * k;
-    if (v4) {  // k % 4 == 0 and X 16-byte aligned
-#pragma unroll
-        for (int q = 0; q < KB; q += 4) {
-            if (q < k) {
-                const float4 t = *reinterpret_cast<const float4*>(p + q);
-                xv[q] = t.x;
-                xv[q + 1] = t.y;
-                xv[q + 2] = t.z;
-                xv[q + 3] = t.w;
-            } else {
-                xv[q] = xv[q + 1] = xv[q + 2] = xv[q + 3] = 0.0f;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < KB; ++q) xv[q] = q < k ? p[q] : 0.0f;
-    }
-}
-
-template <int KB, class V>
-__global__ __launch_bounds__(kBlock) void dia_multi_kernel(
-    int rows, int cols, int len, int ndiag, const int* __restrict__ off,
-    const typename V::T* __restrict__ data, int k, const float* __restrict__ X, bool v4,
-    float* __restrict__ Y, int ldy) {
-    const int r = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (r >= rows) return;
-    float acc[KB];
-#pragma unroll
-    for (int q = 0; q < KB; ++q) acc[q] = 0.0f;
-    for (int j = 0; j < ndiag; ++j) {
-        const float d = V::ld(data + (size_t)j * len + r);
-        const int c = r + off[j];
-        if ((unsigned)c < (unsigned)cols) {
-            float xv[KB];
-            load_xrow<KB>(X, c, k, v4, xv);
-#pragma unroll
-            for (int q = 0; q < KB; ++q) acc[q] = fmaf(d, xv[q], acc[q]);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < KB; ++q)
-        if (q < k) Y[(size_t)q * ldy + r] += acc[q];
-}
-
 // Loads right-hand sides qb .. qb+Q-1 of row c of X (zero where q >= k).
 template <int Q>
 __device__ __forceinline__ void load_xq(const float* __restrict__ X, int c, int k, int qb, bool vec,
                                         float* xv) {
     const float* p = X + (size_t)c * k + qb;
+    if (qb >= k) {  // a lane quad past k (e.g. k = 12 with KB = 16): nothing to load
+#pragma unroll
+        for (int q = 0; q < Q; ++q) xv[q] = 0.0f;
+        return;
+    }
     if constexpr (Q == 4) {
         if (vec) {  // k % 4 == 0, X 16-byte aligned
             const float4 t = *reinterpret_cast<const float4*>(p);
@@ -90,6 +46,39 @@ __device__ __forceinline__ void load_xq(const float* __restrict__ X, int c, int 
     for (int q = 0; q < Q; ++q) xv[q] = qb + q < k ? p[q] : 0.0f;
 }
 
+// DIA: L = KB / Q neighbouring lanes share a row, each with Q = min(KB, 4)
+// right-hand sides, so a wave's X load for one diagonal is one contiguous
+// run of 64 / L rows of X (16-byte pieces).
+template <int KB, class V>
+__global__ __launch_bounds__(kBlock) void dia_multi_kernel(
+    int rows, int cols, int len, int ndiag, const int* __restrict__ off,
+    const typename V::T* __restrict__ data, int k, const float* __restrict__ X, bool vec,
+    float* __restrict__ Y, int ldy) {
+    constexpr int Q = KB < 4 ? KB : 4;
+    constexpr int L = KB / Q;
+    const int t = threadIdx.x;
+    const int qb = (t % L) * Q;
+    const int r = xcd_tile(blockIdx.x, gridDim.x) * (kBlock / L) + t / L;
+    if (r >= rows) return;
+    float acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] = 0.0f;
+#pragma unroll 3
+    for (int j = 0; j < ndiag; ++j) {
+        const float d = V::ld(data + (size_t)j * len + r);
+        const int c = r + off[j];
+        if ((unsigned)c < (unsigned)cols) {
+            float xv[Q];
+            load_xq<Q>(X, c, k, qb, vec, xv);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[q] = fmaf(d, xv[q], acc[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (qb + q < k) Y[(size_t)(qb + q) * ldy + r] += acc[q];
+}
+
 // Sum over the lanes of a group of G (a power of two <= 64) lanes that
 // share lane % L: xor steps G/2 .. L.
 __device__ __forceinline__ float quad_sum(float v, int g, int l) {
@@ -98,12 +87,13 @@ __device__ __forceinline__ float quad_sum(float v, int g, int l) {
 }
 
 // Accumulates a[q] += val[j] * X[col[j]][qb + q] over j = j0, j0 + step, ...
-// below j1, four nonzeros in flight per lane.
-template <int Q, class V>
+// below j1, four nonzeros in flight per lane.  Acc is float, or double for
+// rows longer than kLongRowF64 (the products are exact in double).
+template <int Q, class V, class Acc>
 __device__ __forceinline__ void csr_multi_run(const int* __restrict__ col,
                                               const typename V::T* __restrict__ val, int j0, int j1,
                                               int step, const float* __restrict__ X, int k, int qb,
-                                              bool vec, float* a) {
+                                              bool vec, Acc* a) {
     int j = j0;
     for (; j + 3 * step < j1; j += 4 * step) {
         int c[4];
@@ -119,7 +109,7 @@ __device__ __forceinline__ void csr_multi_run(const int* __restrict__ col,
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int q = 0; q < Q; ++q) a[q] = fmaf(v[u], xv[u][q], a[q]);
+            for (int q = 0; q < Q; ++q) a[q] = fma((Acc)v[u], (Acc)xv[u][q], a[q]);
     }
     for (; j < j1; j += step) {
         const int c = ld_stream(col + j);
@@ -127,7 +117,43 @@ __device__ __forceinline__ void csr_multi_run(const int* __restrict__ col,
         float xv[Q];
         load_xq<Q>(X, c, k, qb, vec, xv);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) a[q] = fmaf(v, xv[q], a[q]);
+        for (int q = 0; q < Q; ++q) a[q] = fma((Acc)v, (Acc)xv[q], a[q]);
+    }
+}
+
+// One row longer than kAdaptiveNnz by the whole work-group: each lane
+// strides the row, the four waves' sums are combined in a fixed order
+// through LDS, and Y[:, r] is updated once per right-hand side.
+template <int KB, class V, class Acc>
+__device__ __forceinline__ void csr_multi_long(const typename V::T* __restrict__ val,
+                                               const int* __restrict__ col, int r, int j0, int j1,
+                                               const float* __restrict__ X, int k, bool vec,
+                                               float* __restrict__ Y, int ldy, Acc (*part)[KB]) {
+    constexpr int Q = KB < 4 ? KB : 4;
+    constexpr int L = KB / Q;
+    const int t = threadIdx.x;
+    const int qb = (t % L) * Q;
+    Acc a[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a[q] = 0;
+    csr_multi_run<Q, V, Acc>(col, val, j0 + t / L, j1, kBlock / L, X, k, qb, vec, a);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        for (int m = kWave >> 1; m >= L; m >>= 1) a[q] += __shfl_xor(a[q], m, kWave);
+    if ((t & (kWave - 1)) < L) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) part[t / kWave][qb + q] = a[q];
+    }
+    __syncthreads();
+    if (t < KB && t < k) {
+        Acc tot = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; ++w) tot += part[w][t];
+        float* yp = Y + (size_t)t * ldy + r;
+        if constexpr (sizeof(Acc) == sizeof(double))
+            *yp = (float)((double)*yp + tot);  // one rounding of the whole row
+        else
+            *yp += tot;
     }
 }
 
@@ -138,8 +164,9 @@ __device__ __forceinline__ void csr_multi_run(const int* __restrict__ col,
 // group of G lanes per row, G the largest power of two with the block's
 // rows fitting the work-group (at least L, at most a wave; rows beyond the
 // work-group loop).  A long row: the whole work-group, the four waves'
-// partial sums combined in a fixed order through LDS.  No product staging,
-// so occupancy is set by registers alone.
+// partial sums combined in a fixed order through LDS, in FP64 for rows
+// longer than kLongRowF64 (products exact, one rounding into y).  No
+// product staging, so occupancy is set by registers alone.
 template <int KB, class V>
 __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
     const int* __restrict__ blocks, const int* __restrict__ rp, const int* __restrict__ col,
@@ -148,6 +175,7 @@ __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
     constexpr int Q = KB < 4 ? KB : 4;
     constexpr int L = KB / Q;
     __shared__ float part[kBlock / kWave][KB];
+    __shared__ double part64[kBlock / kWave][KB];
     const int b = blockIdx.x;
     const int r0 = blocks[b], r1 = blocks[b + 1];
     const int nrows = r1 - r0;
@@ -156,22 +184,10 @@ __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
     const int qb = sub * Q;
     float a[Q];
     if (nrows == 1 && rp[r1] - rp[r0] > launch::kAdaptiveNnz) {
-#pragma unroll
-        for (int q = 0; q < Q; ++q) a[q] = 0.0f;
-        csr_multi_run<Q, V>(col, val, rp[r0] + t / L, rp[r1], kBlock / L, X, k, qb, vec, a);
-#pragma unroll
-        for (int q = 0; q < Q; ++q) a[q] = quad_sum(a[q], kWave, L);
-        if ((t & (kWave - 1)) < L) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) part[t / kWave][qb + q] = a[q];
-        }
-        __syncthreads();
-        if (t < KB && t < k) {
-            float tot = 0.0f;
-#pragma unroll
-            for (int w = 0; w < kBlock / kWave; ++w) tot += part[w][t];
-            Y[(size_t)t * ldy + r0] += tot;
-        }
+        if (rp[r1] - rp[r0] > launch::kLongRowF64)
+            csr_multi_long<KB, V, double>(val, col, r0, rp[r0], rp[r1], X, k, vec, Y, ldy, part64);
+        else
+            csr_multi_long<KB, V, float>(val, col, r0, rp[r0], rp[r1], X, k, vec, Y, ldy, part);
         return;
     }
     int g = kBlock / (nrows > 0 ? nrows : 1);
@@ -181,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
     for (int r = t / g; r < nrows; r += kBlock / g) {  // uniform per group
 #pragma unroll
         for (int q = 0; q < Q; ++q) a[q] = 0.0f;
-        csr_multi_run<Q, V>(col, val, rp[r0 + r] + lane / L, rp[r0 + r + 1], g / L, X, k, qb, vec, a);
+        csr_multi_run<Q, V, float>(col, val, rp[r0 + r] + lane / L, rp[r0 + r + 1], g / L, X, k, qb, vec, a);
 #pragma unroll
         for (int q = 0; q < Q; ++q) a[q] = quad_sum(a[q], g, L);
         if (lane < L) {
@@ -196,19 +212,23 @@ int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 template <int KB, class V>
 hipError_t dia_multi_kb(hipStream_t s, int rows, int cols, int len, int ndiag, const int* off,
-                        const void* data, int k, const float* X, bool v4, float* Y, int ldy) {
-    hipLaunchKernelGGL((dia_multi_kernel<KB, V>), dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s, rows,
-                       cols, len, ndiag, off, static_cast<const typename V::T*>(data), k, X, v4, Y, ldy);
+                        const void* data, int k, const float* X, float* Y, int ldy) {
+    constexpr int Q = KB < 4 ? KB : 4;
+    const bool vec = Q > 1 && k % Q == 0 && ((uintptr_t)X % (Q * sizeof(float))) == 0;
+    hipLaunchKernelGGL((dia_multi_kernel<KB, V>), dim3(ceil_div(rows, kBlock / (KB / Q))), dim3(kBlock), 0,
+                       s, rows, cols, len, ndiag, off, static_cast<const typename V::T*>(data), k, X, vec, Y,
+                       ldy);
     return hipGetLastError();
 }
 
 template <class V>
 hipError_t dia_multi_v(hipStream_t s, int rows, int cols, int len, int ndiag, const int* off,
-                       const void* data, int k, const float* X, bool v4, float* Y, int ldy) {
-    if (k <= 1) return dia_multi_kb<1, V>(s, rows, cols, len, ndiag, off, data, k, X, false, Y, ldy);
-    if (k <= 4) return dia_multi_kb<4, V>(s, rows, cols, len, ndiag, off, data, k, X, v4, Y, ldy);
-    if (k <= 8) return dia_multi_kb<8, V>(s, rows, cols, len, ndiag, off, data, k, X, v4, Y, ldy);
-    return dia_multi_kb<16, V>(s, rows, cols, len, ndiag, off, data, k, X, v4, Y, ldy);
+                       const void* data, int k, const float* X, float* Y, int ldy) {
+    if (k <= 1) return dia_multi_kb<1, V>(s, rows, cols, len, ndiag, off, data, k, X, Y, ldy);
+    if (k <= 2) return dia_multi_kb<2, V>(s, rows, cols, len, ndiag, off, data, k, X, Y, ldy);
+    if (k <= 4) return dia_multi_kb<4, V>(s, rows, cols, len, ndiag, off, data, k, X, Y, ldy);
+    if (k <= 8) return dia_multi_kb<8, V>(s, rows, cols, len, ndiag, off, data, k, X, Y, ldy);
+    return dia_multi_kb<16, V>(s, rows, cols, len, ndiag, off, data, k, X, Y, ldy);
 }
 
 template <int KB, class V>
@@ -239,11 +259,10 @@ hipError_t dia_multi(hipStream_t s, int rows, int cols, int len, int ndiag, cons
                      int dtype, const void* data, int k, const float* X, float* Y, int ldy) {
     if (rows <= 0 || ndiag <= 0 || k <= 0) return hipSuccess;
     if (k > kMultiMax) return hipErrorInvalidValue;
-    const bool v4 = k % 4 == 0 && ((uintptr_t)X & 15) == 0;
     switch (dtype) {
-        case 0: return dia_multi_v<ValF32>(s, rows, cols, len, ndiag, offsets, data, k, X, v4, Y, ldy);
-        case 1: return dia_multi_v<ValBF16>(s, rows, cols, len, ndiag, offsets, data, k, X, v4, Y, ldy);
-        case 2: return dia_multi_v<ValF16>(s, rows, cols, len, ndiag, offsets, data, k, X, v4, Y, ldy);
+        case 0: return dia_multi_v<ValF32>(s, rows, cols, len, ndiag, offsets, data, k, X, Y, ldy);
+        case 1: return dia_multi_v<ValBF16>(s, rows, cols, len, ndiag, offsets, data, k, X, Y, ldy);
+        case 2: return dia_multi_v<ValF16>(s, rows, cols, len, ndiag, offsets, data, k, X, Y, ldy);
         default: return hipErrorInvalidValue;
     }
 }

@@ -239,14 +239,23 @@ __global__ __launch_bounds__(launch::kBlockThreads) void panel_pass2_kernel(
         if (j0 + tid + k * NT < j1) slot[d[k]] = v[k];
     __syncthreads();
     // Each row adds its slots in order (panel order; a run cut at a
-    // sub-chunk boundary keeps its pieces in order).
+    // sub-chunk boundary keeps its pieces in order).  A slot holds at most
+    // one sub-chunk (kPanelSub entries), so every row longer than
+    // kLongRowF64 has more than kLongRowF64 / kPanelSub slots: those rows
+    // add their slots in FP64 and round into y once.
     const int* rs = row_slot + row0 + b;
     for (int i = tid; i < nrows; i += NT) {
         const int s = rs[i], e = rs[i + 1];
         if (s == e) continue;
-        float acc = slot[s];
-        for (int q = s + 1; q < e; ++q) acc += slot[q];
-        y[row0 + i] += acc;
+        if (e - s > launch::kLongRowF64 / launch::kPanelSub) {
+            double acc = slot[s];
+            for (int q = s + 1; q < e; ++q) acc += slot[q];
+            y[row0 + i] = (float)((double)y[row0 + i] + acc);
+        } else {
+            float acc = slot[s];
+            for (int q = s + 1; q < e; ++q) acc += slot[q];
+            y[row0 + i] += acc;
+        }
     }
 }
 

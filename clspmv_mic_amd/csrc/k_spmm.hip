@@ -53,7 +53,11 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
 // quads of four): each A block is loaded once and multiplied by NQ tiles of
 // X, so A is read once per SpMM whatever k is.  Lane t of a group loads row
 // t of the block and, per quad q, column 4q + t of X.
-template <bool BF16, int NQ>
+// LONG (matrices with a block row longer than kLongRowF64 / 4 blocks): the
+// FP32 MFMA accumulators are added into FP64 ones every kSpmmChunk blocks,
+// so a row's rounding error grows with the chunk, not the row length, and
+// y receives one rounding.
+template <bool BF16, int NQ, bool LONG>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int brows, int rows, int cols, int k, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
@@ -95,6 +99,21 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     // Software pipeline: block it + 1's loads are in flight while block it
     // is multiplied.
     s16x4 a_cur, a_nxt, b_cur[NQ], b_nxt[NQ];
+    double acc64[NQ][4];  // used by LONG only
+    if constexpr (LONG) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc64[q][i] = 0.0;
+    }
+    auto carry = [&]() {  // FP32 chunk sums into FP64 (LONG only)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc64[q][i] += (double)acc[q][i];
+            acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    };
     fetch(0, a_cur, b_cur);
     for (int it = 0; it < nmax; ++it) {
         fetch(it + 1, a_nxt, b_nxt);
@@ -103,9 +122,25 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
         a_cur = a_nxt;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) b_cur[q] = b_nxt[q];
+        if constexpr (LONG) {
+            if (it % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
+        }
     }
+    if constexpr (LONG) carry();
     if (!active) return;
     const int r0 = br * 4;
+    if constexpr (LONG) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int xc = q * 4 + t;
+            if (xc >= k) continue;
+            float* yp = Y + (size_t)xc * ldy + r0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (r0 + i < rows) yp[i] = (float)((double)yp[i] + acc64[q][i]);
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int xc = q * 4 + t;
@@ -124,18 +159,22 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
 }
 
 template <bool BF16, int NQ>
-hipError_t launch_nq(hipStream_t s, int brows, int rows, int cols, int k, const int* rp,
+hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                      const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
                      float* Y, int ldy) {
     const long long threads = (long long)brows * 4;
     const long long blocks = (threads + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ>), dim3((unsigned)blocks), dim3(kBlock), 0, s, brows, rows,
-                       cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    if (long_rows)
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, true>), dim3((unsigned)blocks), dim3(kBlock), 0, s, brows,
+                           rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    else
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, brows,
+                           rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
     return hipGetLastError();
 }
 
 template <bool BF16>
-hipError_t launch_k(hipStream_t s, int brows, int rows, int cols, int k, const int* rp,
+hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                     const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
                     float* Y, int ldy) {
     // Up to 32 right-hand sides per launch (8 quads of accumulators).
@@ -145,10 +184,10 @@ hipError_t launch_k(hipStream_t s, int brows, int rows, int cols, int k, const i
         float* Yc = Y + (size_t)c * ldy;
         const int nq = (kc + 3) / 4;
         hipError_t e;
-        if (nq <= 1) e = launch_nq<BF16, 1>(s, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
-        else if (nq <= 2) e = launch_nq<BF16, 2>(s, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
-        else if (nq <= 4) e = launch_nq<BF16, 4>(s, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
-        else e = launch_nq<BF16, 8>(s, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        if (nq <= 1) e = launch_nq<BF16, 1>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else if (nq <= 2) e = launch_nq<BF16, 2>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else if (nq <= 4) e = launch_nq<BF16, 4>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else e = launch_nq<BF16, 8>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -158,13 +197,13 @@ hipError_t launch_k(hipStream_t s, int brows, int rows, int cols, int k, const i
 
 namespace launch {
 
-hipError_t bspmm4(hipStream_t s, bool bf16, int brows, int rows, int cols, int k,
+hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows, int cols, int k,
                   const int* rp, const int* bcol, const unsigned short* val,
                   const unsigned short* X, int ldx, float* Y, int ldy) {
     if (brows <= 0 || k <= 0) return hipSuccess;
     if ((long long)brows * 4 / kBlock + 1 > 0x7FFFFFFFll) return hipErrorInvalidValue;
-    return bf16 ? launch_k<true>(s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy)
-                : launch_k<false>(s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    return bf16 ? launch_k<true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy)
+                : launch_k<false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
 }
 
 }  // namespace launch

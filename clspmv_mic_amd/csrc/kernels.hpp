@@ -17,6 +17,9 @@ hipError_t csr_vector(hipStream_t s, int rows, int lanes, const int* rp,
 // longer row.
 constexpr int kAdaptiveNnz = 1024;
 constexpr int kAdaptiveRows = 256;
+// Rows longer than this are accumulated in FP64 by the 16-bit-value and
+// multi-vector CSR paths, the panel row sums and the block SpMM.
+constexpr int kLongRowF64 = 4096;
 hipError_t csr_adaptive(hipStream_t s, int nblocks, const int* blocks,
                         const int* rp, const int* col, const float* val,
                         const float* x, float* y);
@@ -110,8 +113,10 @@ hipError_t csr_multi(hipStream_t s, int nblocks, const int* blocks, const int* r
 // Y[:, 0:k] += A X[:, 0:k]; A: block rows rp/bcol (element columns, multiples
 // of 4) with 16 BF16 / FP16 values per block, row-major; X: 16-bit, column-
 // major (ldx); Y: FP32 column-major (ldy).  ldx, ldy multiples of 4 and X, Y
-// 16-byte aligned.
-hipError_t bspmm4(hipStream_t s, bool bf16, int brows, int rows, int cols, int k,
+// 16-byte aligned.  long_rows: some block row holds more than kLongRowF64 / 4
+// blocks; the MFMA sums are then carried in FP64 every kSpmmChunk blocks.
+constexpr int kSpmmChunk = 16;
+hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows, int cols, int k,
                   const int* rp, const int* bcol, const unsigned short* val,
                   const unsigned short* X, int ldx, float* Y, int ldy);
 

@@ -533,6 +533,12 @@ int clspmv_upload_dia_ex(clspmv_ctx* ctx, const clspmv_dia_matrix* m, const clsp
     if (dtype == CLSPMV_DT_F32) {
         UPLOAD_CHECK(upload_array(d, m->dia_data, total, &d->val));
     } else {
+        // A DIA row holds at most dia_num entries; the DIA kernels sum in FP32,
+        // so 16-bit values are refused where rows could exceed the FP32 limit.
+        if (m->dia_num > launch::kLongRowF64)
+            UPLOAD_CHECK(fail(CLSPMV_ERR_UNSUPPORTED,
+                              "16-bit DIA with %d diagonals: rows longer than %d need FP64 sums (use CSR)",
+                              m->dia_num, launch::kLongRowF64));
         std::vector<unsigned short> lp(total);
         UPLOAD_CHECK(clspmv_convert_from_f32(m->dia_data, lp.data(), total, dtype));
         UPLOAD_CHECK(upload_array(d, lp.data(), total, &d->lpval));
@@ -762,6 +768,8 @@ int clspmv_upload_bcsr_lowp(clspmv_ctx* ctx, const clspmv_bcsr_matrix* m, int dt
     UPLOAD_CHECK(clspmv_convert_from_f32(blocks.data(), lp.data(), lp.size(), dtype));
     d->dtype = dtype;
     d->brows = nbr;
+    for (int r = 0; r < nbr && !d->long_brows; ++r)
+        d->long_brows = (long long)(m->bcsr_row_ptr[r + 1] - m->bcsr_row_ptr[r]) * 4 > launch::kLongRowF64;
     d->info.bheight = d->info.bwidth = 4;
     d->info.stored = (long long)nb * 16;
     UPLOAD_CHECK(upload_array(d, m->bcsr_row_ptr, (size_t)nbr + 1, &d->rp));
@@ -788,7 +796,7 @@ int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, i
     int st = use_ctx_device(ctx);
     if (st) return st;
     if (k == 0 || I.rows == 0) return CLSPMV_OK;
-    hipError_t e = launch::bspmm4(ctx->stream, A->dtype == CLSPMV_DT_BF16, A->brows, I.rows, I.cols, k,
+    hipError_t e = launch::bspmm4(ctx->stream, A->dtype == CLSPMV_DT_BF16, A->long_brows, A->brows, I.rows, I.cols, k,
                                   A->rp, A->col, A->lpval, static_cast<const unsigned short*>(d_X), ldx,
                                   d_Y, ldy);
     if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "spmm launch failed: %s", hipGetErrorString(e));

@@ -42,6 +42,7 @@ struct clspmv_dmat {
     int len = 0, ndiag = 0, nband = 0, min_off = 0, max_off = 0, max_band = 0;
     // Block formats: block rows, aligned sizes, group lanes.
     int brows = 0, aligned = 0, f4 = 0;
+    bool long_brows = false;  // BSPMM: a block row holds more than kLongRowF64 / 4 blocks
     // x-panel encoding of COO / CSR (pw == 0: absent).  plain == false when
     // the (row, col, val) / CSR arrays were not uploaded.
     int pw = 0, pnwg = 0, pnp = 0, pnb = 0;

@@ -20,33 +20,12 @@
 
 #include "clspmv_serve.h"
 #include "kernels.hpp"
+#include "rng.hpp"
 #include "runtime.hpp"
 
 using namespace clspmv;
 
 namespace {
-
-// SplitMix64 finalizer: a bijective 64-bit mix.
-__host__ __device__ inline unsigned long long mix64(unsigned long long z) {
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-// 24 random bits scaled to [0, 2) minus 1: exact in FP32, so host and
-// device agree bit for bit.
-__host__ __device__ inline float uniform_at(unsigned long long seed, int j, int i) {
-    const unsigned long long h =
-        mix64(seed ^ mix64(((unsigned long long)(unsigned)j << 32) | (unsigned)i));
-    return (float)(h >> 40) * (1.0f / 8388608.0f) - 1.0f;
-}
-
-__device__ inline unsigned short to_bf16(float f) {
-    unsigned u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (x is finite)
-    return (unsigned short)(u >> 16);
-}
 
 __global__ void fill_x_kernel(unsigned long long seed, int k, int cols, int ldx, float* X) {
     const long long n = (long long)k * cols;
@@ -74,14 +53,7 @@ __global__ void fill_x16_kernel(unsigned long long seed, int k, int cols, int ld
          t += (long long)gridDim.x * blockDim.x) {
         const int j = (int)(t / cols), i = (int)(t % cols);
         const float v = uniform_at(seed, j, i);
-        unsigned short h;
-        if (dtype == CLSPMV_DT_BF16) {
-            h = to_bf16(v);
-        } else {
-            const _Float16 f = (_Float16)v;
-            h = __builtin_bit_cast(unsigned short, f);
-        }
-        X[(size_t)j * ldx + i] = h;
+        X[(size_t)j * ldx + i] = dtype == CLSPMV_DT_BF16 ? to_bf16(v) : to_f16(v);
     }
 }
 
@@ -153,7 +125,7 @@ int enqueue_pass(hipStream_t s, const SMatrix& M, int k, bool multi) {
     hipLaunchKernelGGL(zero_kernel, dim3(grid_for(ny)), dim3(256), 0, s, ny, M.Y);
     CLSPMV_HIP(hipGetLastError());
     if (M.lowp) {
-        CLSPMV_HIP(launch::bspmm4(s, M.A->dtype == CLSPMV_DT_BF16, M.A->brows, M.rows, M.cols, k,
+        CLSPMV_HIP(launch::bspmm4(s, M.A->dtype == CLSPMV_DT_BF16, M.A->long_brows, M.A->brows, M.rows, M.cols, k,
                                   M.A->rp, M.A->col, M.A->lpval,
                                   static_cast<const unsigned short*>(M.X), M.ldx, M.Y, M.ldy));
         return CLSPMV_OK;

@@ -195,6 +195,40 @@ int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, i
 int clspmv_convert_from_f32(const float* src, void* dst, size_t n, int dtype);
 int clspmv_convert_to_f32(const void* src, float* dst, size_t n, int dtype);
 
+/* ---- verification and per-matrix error (src/util.cpp:85-131) ------------
+ * The reference recomputes y = A x serially on the host (spmv_only,
+ * src/util.cpp:85) and compares with two_vec_compare (:93-131), from
+ * src/eval.cpp:79-97.  Here the check runs on the GPU in FP64 over an FP32
+ * CSR reference `ref` (clspmv_upload_csr of the original values; it must
+ * hold its own arrays: panel off or CLSPMV_PANEL_BOTH): per row
+ *     y64 = y0 + sum double(a) * double(x),  bound = |y0| + sum |a x|,
+ *     e   = |y - y64| / bound   (0 / +inf when bound is 0; NaN -> +inf),
+ * the row-scaled metric of the FP32 tolerance.  Blocks until done; the
+ * report is the same on every run (fixed reduction order).
+ *
+ * clspmv_verify_spmv: scores a result d_y of y = y0 + A x that the caller
+ *   computed (d_y0 may be NULL for y0 = 0).
+ * clspmv_dmat_error_report: the error of a device matrix A (any format and
+ *   value type: FP32, BF16 or FP16 storage) against the FP64 SpMV of `ref`
+ *   (the same matrix before rounding), on x_i = uniform(seed, 0, i) (the
+ *   values of clspmv_request_x): y = A x by clspmv_spmv with the automatic
+ *   method, or by clspmv_spmm with k = 1 for CLSPMV_FMT_BSPMM (x rounded to
+ *   the matrix's 16-bit type, so that rounding is part of the error). */
+typedef struct {
+    double    max_scaled_error;  /* max_i e_i                             */
+    double    rms_scaled_error;  /* sqrt(mean_i e_i^2)                    */
+    double    max_abs_error;     /* max_i |y_i - y64_i|                   */
+    int       worst_row;         /* first row reaching max_scaled_error   */
+    int       rows;
+    long long rows_over_tol;     /* rows with e_i > tol                   */
+    double    tol;
+} clspmv_error_report;
+
+int clspmv_verify_spmv(clspmv_ctx* ctx, const clspmv_dmat* ref, const float* d_x, const float* d_y0,
+                       const float* d_y, double tol, clspmv_error_report* rep);
+int clspmv_dmat_error_report(clspmv_ctx* ctx, const clspmv_dmat* A, const clspmv_dmat* ref,
+                             unsigned long long seed, double tol, clspmv_error_report* rep);
+
 /* ---- bandwidth probe (src/mem_bandwidth.cpp bandwidth_test :14) ---------
  * Copies `bytes` (rounded down to 16) between two fresh device buffers
  * `ntimes` times after one warm-up copy and returns the average copy time

@@ -307,7 +307,7 @@ MULTI_CASES = ["rand_small", "rand_dyadic", "rand_long_rows", "rand_long_dyadic"
 @pytest.mark.parametrize("name", MULTI_CASES)
 @pytest.mark.parametrize("fmt", ["dia", "csr"])
 @pytest.mark.parametrize("dtype", [cs.DT_F32, cs.DT_BF16, cs.DT_F16])
-@pytest.mark.parametrize("k", [1, 3, 4, 8, 16])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 12, 16])
 def test_spmv_multi(ctx, oracle, name, fmt, dtype, k):
     """One pass for k right-hand sides (X row-major, Y column-major) equals k
     oracle SpMVs; 16-bit values are compared with the values as stored."""
