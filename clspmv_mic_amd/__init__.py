@@ -152,7 +152,8 @@ class DmatInfo(C.Structure):
     _fields_ = [("format", C.c_int), ("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int),
                 ("stored", C.c_longlong), ("device_bytes", C.c_size_t), ("bheight", C.c_int),
                 ("bwidth", C.c_int), ("slice_height", C.c_int), ("panel_width", C.c_int),
-                ("panel_active", C.c_int), ("plain", C.c_int), ("dtype", C.c_int)]
+                ("panel_active", C.c_int), ("plain", C.c_int), ("dtype", C.c_int),
+                ("panel_groups", C.c_int), ("panel_subchunks", C.c_int), ("panel_blocks", C.c_int)]
 
 
 class DriverReport(C.Structure):
@@ -244,7 +245,7 @@ for _n, _T in [("csr", CsrMatrix), ("coo", CooMatrix), ("ell", EllMatrix), ("sel
 _sig("clspmv_dmat_free", C.c_int, VP)
 _sig("clspmv_dmat_get_info", C.c_int, VP, P(DmatInfo))
 class UploadOpts(C.Structure):
-    _fields_ = [("panel_width", C.c_int), ("dtype", C.c_int)]
+    _fields_ = [("panel_width", C.c_int), ("dtype", C.c_int), ("panel_group", C.c_int)]
 
 
 _sig("clspmv_upload_coo_ex", C.c_int, VP, P(CooMatrix), P(UploadOpts), P(VP))
@@ -752,17 +753,19 @@ class DeviceMatrix:
     (DT_F32, DT_BF16, DT_F16) -- clspmv_upload_{coo,csr,dia}_ex.  Both None:
     the plain upload call."""
 
-    def __init__(self, ctx: Context, m, panel: Optional[int] = None, dtype: Optional[int] = None):
+    def __init__(self, ctx: Context, m, panel: Optional[int] = None, dtype: Optional[int] = None,
+                 group: int = 0):
         s = m.s if isinstance(m, Owned) else (m.c if isinstance(m, Coo) else m)
         h = VP()
         name = _UPLOAD[type(s)]
-        if panel is not None or dtype is not None:
+        if panel is not None or dtype is not None or group:
             if name not in ("coo", "csr", "dia"):
                 raise ValueError("encoding options exist for COO, CSR and DIA only")
             default_pw = PANEL_AUTO if name == "coo" else 0
             if name == "csr" and dtype not in (None, DT_F32) and panel is None:
                 default_pw = PANEL_AUTO
-            opts = UploadOpts(default_pw if panel is None else int(panel), DT_F32 if dtype is None else dtype)
+            opts = UploadOpts(default_pw if panel is None else int(panel), DT_F32 if dtype is None else dtype,
+                              int(group))
             check(getattr(lib, f"clspmv_upload_{name}_ex")(ctx.h, C.byref(s), C.byref(opts), C.byref(h)),
                   "upload")
         else:

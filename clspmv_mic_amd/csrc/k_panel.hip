@@ -68,8 +68,12 @@ struct SubChunk {
     f32x4 v;   // four values
 };
 
+// Reduces one sub-chunk (4 entries per lane) to its segment sums.  xs holds
+// the panel's x and a 0 at index pad (the panel width): padding entries
+// (column word 0x7FFF >= pad) gather it, so every lane reads x with no
+// branch and the four LDS reads are in flight together.
 __device__ __forceinline__ void seg_scan_write(const SubChunk& q, const float* __restrict__ xs,
-                                               int lane, float* __restrict__ out) {
+                                               unsigned pad, int lane, float* __restrict__ out) {
     const unsigned w[4] = {q.c.x & 0xFFFFu, q.c.x >> 16, q.c.y & 0xFFFFu, q.c.y >> 16};
     const float v[4] = {q.v.x, q.v.y, q.v.z, q.v.w};
     int f[4];
@@ -77,39 +81,29 @@ __device__ __forceinline__ void seg_scan_write(const SubChunk& q, const float* _
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         f[k] = (int)(w[k] >> 15);
-        const unsigned c = w[k] & 0x7FFFu;
-        const bool real = c != launch::kPanelPad;
-        const float xv = xs[real ? c : 0u];
-        p[k] = real ? v[k] * xv : 0.0f;
+        p[k] = v[k] * xs[min(w[k] & 0x7FFFu, pad)];
     }
     if (lane == 0) f[0] = 1;  // every sub-chunk opens a segment
     // Segment numbering: flags of earlier lanes, then this lane's own.
     int before = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) before += lanes_below(__ballot(f[k]));
-    const int nf = f[0] + f[1] + f[2] + f[3];
-
-    // Runs inside the lane: head (before the first flag), whole segments,
-    // tail (from the last flag on).
-    float head = 0.0f, run = 0.0f;
-    int id = before - 1;  // segment of the current run
-    bool seen = false;
+    // Running sums inside the lane, restarted at each flag.
+    float r[4];
+    r[0] = p[0];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (f[k]) {
-            if (seen) out[id] = run;  // a segment that starts and ends here
-            else head = run;
-            seen = true;
-            ++id;
-            run = p[k];
-        } else {
-            run += p[k];
-        }
-    }
-    if (!seen) head = run;
+    for (int k = 1; k < 4; ++k) r[k] = f[k] ? p[k] : r[k - 1] + p[k];
+    const int c0 = f[0], c1 = c0 + f[1], c2 = c1 + f[2], nf = c2 + f[3];
+    // Segments that open and close inside the lane.
+    if (f[1] && c0) out[before + c0 - 1] = r[0];
+    if (f[2] && c1) out[before + c1 - 1] = r[1];
+    if (f[3] && c2) out[before + c2 - 1] = r[2];
+    // The lane's head: its entries before its first flag.
+    const float head = f[0] ? 0.0f : f[1] ? r[0] : f[2] ? r[1] : f[3] ? r[2] : r[3];
     // Segmented inclusive scan over lanes of the amount each lane passes on:
-    // its tail when it holds a flag, else its whole sum.
-    float s = nf ? run : head;
+    // its tail from the last flag, or its whole sum when it has none -- r[3]
+    // either way.
+    float s = r[3];
     int g = nf ? 1 : 0;
     seg_step<kRowShr1, 0xF>(s, g);
     seg_step<kRowShr2, 0xF>(s, g);
@@ -161,6 +155,7 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
             if (i < w4) *reinterpret_cast<f32x4*>(xs + i) = v[k];
         }
         for (int i = w4 + tid; i < w; i += NT) xs[i] = x[xb + i];
+        if (tid == 0) xs[pw] = 0.0f;  // the padding entries' x
     } else {
         float v[4 * kStage];
 #pragma unroll
@@ -173,6 +168,7 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
             const int i = tid + k * NT;
             if (i < w) xs[i] = v[k];
         }
+        if (tid == 0) xs[pw] = 0.0f;  // the padding entries' x
     }
     __syncthreads();
 
@@ -209,7 +205,7 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int c = sc + u * nw;
-            if (c < sc1) seg_scan_write(cur[u], xs, lane, seg_val + s_seg[c - sc0]);
+            if (c < sc1) seg_scan_write(cur[u], xs, (unsigned)pw, lane, seg_val + s_seg[c - sc0]);
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
@@ -222,10 +218,14 @@ __global__ __launch_bounds__(launch::kBlockThreads) void panel_pass2_kernel(
     const float* __restrict__ seg_val, float* __restrict__ y) {
     constexpr int NT = launch::kBlockThreads, K = launch::kBlockItems;
     __shared__ float slot[launch::kBlockSegsMax];
+    constexpr int R = launch::kBlockRowsMax / NT;  // rows per thread at most
+    static_assert(R * NT == launch::kBlockRowsMax, "rows of a block must split evenly over the threads");
     const int b = blockIdx.x, tid = threadIdx.x;
     const int row0 = blk_row[b], nrows = blk_row[b + 1] - row0;
     const int j0 = blk_seg[b], j1 = blk_seg[b + 1];
-    // Every segment of the block is loaded before any is placed.
+    // Every load of the block -- its segments, the slot range and y of each
+    // of this thread's rows -- is issued before the first is used, so the
+    // work-group waits for one memory latency.
     unsigned short d[K];
     float v[K];
 #pragma unroll
@@ -233,6 +233,17 @@ __global__ __launch_bounds__(launch::kBlockThreads) void panel_pass2_kernel(
         const int j = j0 + tid + k * NT;
         d[k] = j < j1 ? seg_dst[j] : 0;
         v[k] = j < j1 ? seg_val[j] : 0.0f;
+    }
+    const int* rs = row_slot + row0 + b;
+    int s0[R], s1[R];
+    float yv[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * NT;
+        const bool live = i < nrows;
+        s0[k] = live ? rs[i] : 0;
+        s1[k] = live ? rs[i + 1] : 0;
+        yv[k] = live ? y[row0 + i] : 0.0f;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -243,18 +254,19 @@ __global__ __launch_bounds__(launch::kBlockThreads) void panel_pass2_kernel(
     // one sub-chunk (kPanelSub entries), so every row longer than
     // kLongRowF64 has more than kLongRowF64 / kPanelSub slots: those rows
     // add their slots in FP64 and round into y once.
-    const int* rs = row_slot + row0 + b;
-    for (int i = tid; i < nrows; i += NT) {
-        const int s = rs[i], e = rs[i + 1];
-        if (s == e) continue;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * NT;
+        const int s = s0[k], e = s1[k];
+        if (i >= nrows || s == e) continue;
         if (e - s > launch::kLongRowF64 / launch::kPanelSub) {
             double acc = slot[s];
             for (int q = s + 1; q < e; ++q) acc += slot[q];
-            y[row0 + i] = (float)((double)y[row0 + i] + acc);
+            y[row0 + i] = (float)((double)yv[k] + acc);
         } else {
             float acc = slot[s];
             for (int q = s + 1; q < e; ++q) acc += slot[q];
-            y[row0 + i] += acc;
+            y[row0 + i] = yv[k] + acc;
         }
     }
 }
@@ -266,7 +278,7 @@ namespace launch {
 hipError_t panel_prepare() {
     // Pass 1 holds up to kPanelMax floats of x in dynamic LDS, beyond the
     // default limit: raise it once per device, outside any graph capture.
-    const int lds = (int)(sizeof(float) * kPanelMax);
+    const int lds = (int)(sizeof(float) * (kPanelMax + 1));
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_pass1_kernel<kPanelThreads, ValF32>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e == hipSuccess)
@@ -285,7 +297,7 @@ hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
                       const int* row_slot, const unsigned short* seg_dst, float* y) {
     if (nwg <= 0 || nblocks <= 0) return hipSuccess;
     if (pw <= 0 || pw > kPanelMax) return hipErrorInvalidValue;
-    const size_t lds = sizeof(float) * (size_t)pw;
+    const size_t lds = sizeof(float) * ((size_t)pw + 1);  // x panel + the padding entries' 0
     const int4* wg4 = reinterpret_cast<const int4*>(wg);
     const unsigned short* p16 = static_cast<const unsigned short*>(pval);
     switch (dtype) {

@@ -52,7 +52,7 @@ constexpr int kPanelSub = 256;        // entries per wave-sized sub-chunk
 constexpr int kPanelMax = 32760;      // largest panel width (floats)
 constexpr unsigned kPanelPad = 0x7FFF;  // column code of padding entries
 constexpr int kPanelThreads = 1024;   // pass-1 work-group size
-constexpr int kPanelScPerWg = 128;    // most sub-chunks of one pass-1 work-group
+constexpr int kPanelScPerWg = 512;    // most sub-chunks of one pass-1 work-group
 constexpr int kPanelsMax = 2048;      // most panels of one matrix (bounds the encoder's tables)
 constexpr int kBlockRowsMax = 4096;   // rows of a pass-2 row block
 constexpr int kBlockThreads = 1024;   // pass-2 work-group size

@@ -20,8 +20,27 @@ int auto_panel_width(int cols, long long nnz) {
     return pw;
 }
 
+int panel_group_size(const std::vector<int>& sc, int slots, int pw) {
+    // Staging a panel costs about as much as this many sub-chunks (pw floats
+    // from L2 against 1.5 KB of encoded entries per sub-chunk from HBM).
+    const long long stage = std::max(8, pw / 1024);
+    long long best = -1;
+    int bw = launch::kPanelScPerWg;
+    for (int W = 8; W <= launch::kPanelScPerWg; W += 8) {
+        long long n = 0;
+        for (int c : sc) n += (c + W - 1) / W;
+        const long long rounds = (n + slots - 1) / std::max(slots, 1);
+        const long long cost = rounds * (W + stage);
+        if (best < 0 || cost < best) {
+            best = cost;
+            bw = W;
+        }
+    }
+    return bw;
+}
+
 int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* row,
-                const int* col, const float* val, int pw, int dtype) {
+                const int* col, const float* val, int pw, int dtype, int group) {
     constexpr int kSub = launch::kPanelSub;
     if (pw <= 0 || pw % 8 || pw > launch::kPanelMax)
         return fail(CLSPMV_ERR_INVALID, "panel width %d: need a multiple of 8 in [8, %d]", pw,
@@ -32,14 +51,24 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
         return fail(CLSPMV_ERR_UNSUPPORTED, "%d panels of width %d: at most %d panels", P, pw,
                     launch::kPanelsMax);
 
+    // Compute units of the device: both passes size their work-groups to
+    // fill them in whole rounds.
+    int ncu = 256;
+    {
+        hipDeviceProp_t prop{};
+        if (hipGetDeviceProperties(&prop, d->device) == hipSuccess && prop.multiProcessorCount > 0)
+            ncu = prop.multiProcessorCount;
+    }
+
     // Row blocks for pass 2: at most kBlockRowsMax rows and, bounding each
     // row's segments by its per-panel runs (a run of L > 1 entries becomes
-    // at most L/256 + 2 segments), at most kBlockSegsMax segments.
+    // at most L/256 + 2 segments), at most `cap` segments (a longer row
+    // alone, up to kBlockSegsMax).  Two pass-2 work-groups run per CU; cap
+    // splits the segments evenly over whole rounds of them.
     if (rows >= (1 << 30)) return fail(CLSPMV_ERR_UNSUPPORTED, "panel encoding needs rows < 2^30");
-    std::vector<int> blk_row(1, 0);
+    std::vector<int> rbound(rows);
+    long long total_bound = 0;
     {
-        long long bound_sum = 0;
-        int r0 = 0;
         long long i = 0;
         for (int r = 0; r < rows; ++r) {
             long long bound = 0;
@@ -54,14 +83,30 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
             }
             if (bound > launch::kBlockSegsMax)
                 return fail(CLSPMV_ERR_UNSUPPORTED, "row %d is too long for the panel encoding", r);
-            if (r - r0 == launch::kBlockRowsMax || bound_sum + bound > launch::kBlockSegsMax) {
+            rbound[r] = (int)bound;
+            total_bound += bound;
+        }
+        if (i != nnz) return fail(CLSPMV_ERR_INVALID, "panel encoding needs row-sorted entries");
+    }
+    long long cap = launch::kBlockSegsMax;
+    {
+        const long long slots = 2ll * ncu;
+        const long long rounds = std::max(1ll, (total_bound + slots * cap - 1) / (slots * cap));
+        // 10 % headroom: greedy packing leaves blocks short of the cap.
+        cap = std::min<long long>(cap, std::max(1024ll, total_bound * 11 / (10 * rounds * slots) + 1));
+    }
+    std::vector<int> blk_row(1, 0);
+    {
+        long long bound_sum = 0;
+        int r0 = 0;
+        for (int r = 0; r < rows; ++r) {
+            if (r > r0 && (r - r0 == launch::kBlockRowsMax || bound_sum + rbound[r] > cap)) {
                 blk_row.push_back(r);
                 r0 = r;
                 bound_sum = 0;
             }
-            bound_sum += bound;
+            bound_sum += rbound[r];
         }
-        if (i != nnz) return fail(CLSPMV_ERR_INVALID, "panel encoding needs row-sorted entries");
         blk_row.push_back(rows);
     }
     const int nb = (int)blk_row.size() - 1;
@@ -195,8 +240,17 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
         }
     }
 
-    // Pass-1 work-groups: runs of at most W sub-chunks of one panel.
-    const int W = std::max(32, std::min(launch::kPanelScPerWg, pw / kSub));
+    // Pass-1 work-groups: runs of at most W sub-chunks of one panel.  Each
+    // holds its panel in LDS, so few fit on a CU at once: W is chosen so
+    // that the work-groups fill the CUs in whole rounds.
+    int W = group;
+    if (W <= 0) {
+        const long long lds = 4ll * pw + 8ll * (launch::kPanelScPerWg + 1);
+        const int per_cu = (int)std::max(1ll, std::min(2ll, (160ll << 10) / lds));
+        std::vector<int> scp(P);
+        for (int p = 0; p < P; ++p) scp[p] = g_sc[(size_t)(p + 1) * nb] - g_sc[(size_t)p * nb];
+        W = panel_group_size(scp, ncu * per_cu, pw);
+    }
     std::vector<int> wg;
     for (int p = 0; p < P; ++p) {
         const int b0 = g_sc[(size_t)p * nb], b1 = g_sc[(size_t)(p + 1) * nb];
@@ -215,6 +269,7 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
     }
     d->pw = pw;
     d->pnwg = (int)(wg.size() / 4);
+    d->pnsc = g_sc[G];
     d->pnp = P;
     d->pnb = nb;
     int st = upload_array(d, wg.data(), wg.size(), &d->pwg);

@@ -358,7 +358,7 @@ static int add_panel(clspmv_dmat* d, const clspmv_upload_opts& o, const int* row
         if (pw == 0 && o.dtype != CLSPMV_DT_F32) pw = std::min(launch::kPanelMax, round_up(I.cols, 8));
     }
     if (pw == 0) return CLSPMV_OK;
-    int st = build_panel(d, I.rows, I.cols, I.nnz, row, col, val, pw, o.dtype);
+    int st = build_panel(d, I.rows, I.cols, I.nnz, row, col, val, pw, o.dtype, o.panel_group);
     if (st == CLSPMV_ERR_UNSUPPORTED && o.panel_width < 0 && o.dtype == CLSPMV_DT_F32)
         return CLSPMV_OK;  // stay plain
     if (st) return st;
@@ -370,6 +370,9 @@ static int add_panel(clspmv_dmat* d, const clspmv_upload_opts& o, const int* row
 static int check_opts(const clspmv_upload_opts& o, bool csr) {
     if (o.dtype != CLSPMV_DT_F32 && o.dtype != CLSPMV_DT_BF16 && o.dtype != CLSPMV_DT_F16)
         return fail(CLSPMV_ERR_INVALID, "dtype %d", o.dtype);
+    if (o.panel_group != 0 && (o.panel_group < 8 || o.panel_group > launch::kPanelScPerWg))
+        return fail(CLSPMV_ERR_INVALID, "panel_group %d: need 0 or [8, %d]", o.panel_group,
+                    launch::kPanelScPerWg);
     const int pw = o.panel_width;
     if (o.dtype != CLSPMV_DT_F32 && (pw == CLSPMV_PANEL_BOTH || (pw == 0 && !csr)))
         return fail(CLSPMV_ERR_UNSUPPORTED, "16-bit COO values need the panel encoding only");
@@ -820,6 +823,9 @@ int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out) {
     *out = m->info;
     out->panel_width = m->pw;
     out->panel_active = m->use_panel ? 1 : 0;
+    out->panel_groups = m->pw ? m->pnwg : 0;
+    out->panel_subchunks = m->pw ? m->pnsc : 0;
+    out->panel_blocks = m->pw ? m->pnb : 0;
     out->plain = m->plain ? 1 : 0;
     out->dtype = m->dtype;
     return CLSPMV_OK;

@@ -45,7 +45,7 @@ struct clspmv_dmat {
     bool long_brows = false;  // BSPMM: a block row holds more than kLongRowF64 / 4 blocks
     // x-panel encoding of COO / CSR (pw == 0: absent).  plain == false when
     // the (row, col, val) / CSR arrays were not uploaded.
-    int pw = 0, pnwg = 0, pnp = 0, pnb = 0;
+    int pw = 0, pnwg = 0, pnp = 0, pnb = 0, pnsc = 0;
     int* pwg = nullptr;  // int4 per pass-1 work-group
     unsigned short* pcol = nullptr;
     float* pval = nullptr;
@@ -117,9 +117,14 @@ void release_array(clspmv_dmat* m, T** p) {
 
 // x-panel encoding (k_panel.hip) of row-sorted entries, built on the host
 // and uploaded into `d`.  pw: panel width (multiple of 8, at most
-// launch::kPanelMax).
+// launch::kPanelMax); group: sub-chunks per pass-1 work-group, 0 = sized by
+// panel_group_size.
 int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* row,
-                const int* col, const float* val, int pw, int dtype);
+                const int* col, const float* val, int pw, int dtype, int group);
+// Sub-chunks per pass-1 work-group for panels holding sc[p] sub-chunks: the
+// size whose work-groups finish soonest when `slots` of them run at once,
+// each costing its sub-chunks plus the staging of its panel.
+int panel_group_size(const std::vector<int>& sc, int slots, int pw);
 // Panel width the encoding rule picks for a matrix, or 0 when panels do not
 // pay (too few nonzeros per panel to amortize staging x in LDS).
 int auto_panel_width(int cols, long long nnz);

@@ -87,6 +87,9 @@ typedef struct {
     int    panel_active;    /* 1 when launches use the panel encoding   */
     int    plain;           /* 1 when the format's own arrays are held  */
     int    dtype;           /* enum clspmv_dtype of the stored values    */
+    int    panel_groups;    /* panel encoding: pass-1 work-groups        */
+    int    panel_subchunks; /* panel encoding: 256-entry sub-chunks      */
+    int    panel_blocks;    /* panel encoding: pass-2 row blocks         */
 } clspmv_dmat_info;
 
 /* Upload a host matrix (the reference's cpy_<fmt>_mat,
@@ -137,6 +140,9 @@ enum clspmv_dtype { CLSPMV_DT_F32 = 0, CLSPMV_DT_BF16 = 1, CLSPMV_DT_F16 = 2 };
 typedef struct {
     int panel_width;
     int dtype;
+    int panel_group;  /* sub-chunks (256 entries) per pass-1 work-group: 8..512,
+                         or 0 to size the work-groups so that they fill the
+                         GPU's CUs in whole rounds */
 } clspmv_upload_opts;
 int clspmv_upload_coo_ex(clspmv_ctx* ctx, const clspmv_coo_matrix* m,
                          const clspmv_upload_opts* opts, clspmv_dmat** out);
