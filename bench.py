@@ -214,11 +214,12 @@ def run_serve(args):
             d = cs.DeviceMatrix(ctx, cs.coo2dia(A.copy()), dtype=dt)
             kind = "stencil"
         else:
-            # CSR: single right-hand sides run the x-panel encoding (FP32)
-            # or the 16-bit CSR kernel, several run one multi-vector pass.
+            # CSR with both encodings: single right-hand sides run the
+            # x-panel encoding, several run whichever of k panel SpMVs or one
+            # multi-vector pass the server measured faster.
             A = matrices.power_law(m, seed=100 + i, band=())
             csr = cs.coo2csr(A.copy())
-            d = cs.DeviceMatrix(ctx, csr, panel=0 if bf16 else cs.PANEL_BOTH, dtype=dt)
+            d = cs.DeviceMatrix(ctx, csr, panel=cs.PANEL_BOTH, dtype=dt)
             kind = "powerlaw"
         if bf16:  # per-matrix error of the 16-bit storage against FP64 on the GPU
             ref = cs.DeviceMatrix(ctx, csr if kind == "powerlaw" else cs.coo2csr(A.copy()), panel=0)

@@ -316,6 +316,7 @@ void drop_encoding(clspmv_dmat* d, bool panel) {
     release_array(d, &d->rp);
     release_array(d, &d->col);
     release_array(d, &d->val);
+    release_array(d, &d->lpval);
     release_array(d, &d->carry_row);
     release_array(d, &d->carry_val);
     release_array(d, &d->blocks);

@@ -352,14 +352,17 @@ static int add_panel(clspmv_dmat* d, const clspmv_upload_opts& o, const int* row
                      const float* val) {
     const clspmv_dmat_info& I = d->info;
     int pw = o.panel_width;
+    // The plain arrays can serve alone: FP32 values, or 16-bit CSR values
+    // kept beside the panels (BOTH).
+    const bool plain_ok = o.dtype == CLSPMV_DT_F32 ||
+                          (o.panel_width == CLSPMV_PANEL_BOTH && I.format == CLSPMV_FMT_CSR);
     if (pw == CLSPMV_PANEL_AUTO || pw == CLSPMV_PANEL_BOTH) {
         pw = auto_panel_width(I.cols, I.nnz);
-        // 16-bit values exist only in the panel encoding.
-        if (pw == 0 && o.dtype != CLSPMV_DT_F32) pw = std::min(launch::kPanelMax, round_up(I.cols, 8));
+        if (pw == 0 && !plain_ok) pw = std::min(launch::kPanelMax, round_up(I.cols, 8));
     }
     if (pw == 0) return CLSPMV_OK;
     int st = build_panel(d, I.rows, I.cols, I.nnz, row, col, val, pw, o.dtype, o.panel_group);
-    if (st == CLSPMV_ERR_UNSUPPORTED && o.panel_width < 0 && o.dtype == CLSPMV_DT_F32)
+    if (st == CLSPMV_ERR_UNSUPPORTED && o.panel_width < 0 && plain_ok)
         return CLSPMV_OK;  // stay plain
     if (st) return st;
     d->dtype = o.dtype;
@@ -374,7 +377,7 @@ static int check_opts(const clspmv_upload_opts& o, bool csr) {
         return fail(CLSPMV_ERR_INVALID, "panel_group %d: need 0 or [8, %d]", o.panel_group,
                     launch::kPanelScPerWg);
     const int pw = o.panel_width;
-    if (o.dtype != CLSPMV_DT_F32 && (pw == CLSPMV_PANEL_BOTH || (pw == 0 && !csr)))
+    if (o.dtype != CLSPMV_DT_F32 && !csr && (pw == CLSPMV_PANEL_BOTH || pw == 0))
         return fail(CLSPMV_ERR_UNSUPPORTED, "16-bit COO values need the panel encoding only");
     if (pw == 0 || pw == CLSPMV_PANEL_AUTO || pw == CLSPMV_PANEL_BOTH) return CLSPMV_OK;
     if (pw < 8 || pw % 8 || pw > launch::kPanelMax)
@@ -417,7 +420,7 @@ int clspmv_upload_csr_ex(clspmv_ctx* ctx, const clspmv_csr_matrix* m,
     d->lanes = pow2_at_least((int)std::ceil(avg), 64);
     UPLOAD_CHECK(upload_array(d, m->csr_row_ptr, (size_t)rows + 1, &d->rp));
     UPLOAD_CHECK(upload_array(d, m->csr_col_id, nnz, &d->col));
-    if (o.dtype != CLSPMV_DT_F32 && o.panel_width == 0) {
+    if (o.dtype != CLSPMV_DT_F32) {
         // Own arrays with 16-bit values (run by the multi-vector kernel).
         std::vector<unsigned short> lp(nnz);
         UPLOAD_CHECK(clspmv_convert_from_f32(m->csr_data, lp.data(), (size_t)nnz, o.dtype));

@@ -126,10 +126,12 @@ int clspmv_dmat_free(clspmv_dmat* m);
  *
  * dtype: CLSPMV_DT_F32 (the reference's values), or CLSPMV_DT_BF16 /
  * CLSPMV_DT_F16: values rounded to 16 bits (nearest even) and widened to
- * FP32 in the kernels, which accumulate in FP32 as before.  16-bit values
- * exist for DIA (every method), for the COO / CSR panel encoding (AUTO then
- * always uses panels; BOTH is refused) and for CSR's own arrays
- * (panel_width 0; every method runs the multi-vector kernel with k = 1).
+ * FP32 in the kernels, which accumulate in FP32 as before (FP64 for rows
+ * longer than 4096 entries on the CSR and panel paths).  16-bit values
+ * exist for DIA (every method, at most 4096 diagonals), for the COO / CSR
+ * panel encoding (COO: AUTO then always uses panels, BOTH is refused) and
+ * for CSR's own arrays (every method runs the multi-vector kernel with
+ * k = 1), alone (panel_width 0) or beside the panels (BOTH).
  *
  * clspmv_upload_coo is clspmv_upload_coo_ex with {CLSPMV_PANEL_AUTO, F32},
  * clspmv_upload_csr is clspmv_upload_csr_ex with {0, F32}, clspmv_upload_dia

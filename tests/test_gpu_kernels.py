@@ -351,6 +351,39 @@ def test_csr_16bit_values(ctx, oracle, name, dtype):
         _run(ctx, oracle, A, cs.coo2csr(A.copy()), method, panel=0, dtype=dtype)
 
 
+@pytest.mark.parametrize("dtype", [cs.DT_BF16, cs.DT_F16])
+def test_csr_16bit_both_encodings(ctx, oracle, dtype):
+    """16-bit CSR values in both encodings: the panel SpMV, the plain SpMV
+    after clspmv_dmat_set_panel(0), and a multi-vector pass each match the
+    oracle on the values as stored."""
+    A = matrices.random_sparse(20000, 30000, 0.0005, seed=12, long_rows=4, long_len=5000)
+    vals = cs.convert_to_f32(cs.convert_from_f32(A.val, dtype), dtype)
+    dA = cs.DeviceMatrix(ctx, cs.coo2csr(A.copy()), panel=cs.PANEL_BOTH, dtype=dtype)
+    assert dA.info.plain == 1 and dA.info.panel_active == 1 and dA.info.dtype == dtype
+    x = matrices.uniform_vector(A.width, seed=3)
+    y0 = matrices.uniform_vector(A.height, seed=4)
+    y64, bound = oracle.spmv_f64(A.row, A.col, vals, x, A.height, y0)
+    dx = cs.DeviceVector(ctx, A.width, x)
+    for panel in (True, False):
+        dA.set_panel(panel)
+        assert dA.info.panel_active == int(panel)
+        y = cs.DeviceVector(ctx, A.height, y0)
+        dA.spmv(dx, y)
+        ctx.sync()
+        err, worst = oracle.scaled_error(y.download(), y64, bound)
+        assert err <= TOL, f"panel={panel}: {err:.3e} at row {worst}"
+    k = 4
+    X = np.random.default_rng(5).uniform(-1, 1, size=(A.width, k)).astype(np.float32)
+    dY = cs.DeviceVector(ctx, k * A.height, np.zeros(k * A.height, np.float32))
+    dA.spmv_multi(k, cs.DeviceVector(ctx, X.size, X.reshape(-1)), dY, A.height)
+    ctx.sync()
+    Y = dY.download().reshape(k, A.height)
+    for q in range(k):
+        y64q, bq = oracle.spmv_f64(A.row, A.col, vals, X[:, q].copy(), A.height)
+        err, worst = oracle.scaled_error(Y[q].copy(), y64q, bq)
+        assert err <= TOL, f"rhs {q}: {err:.3e} at row {worst}"
+
+
 def test_spmv_multi_refuses_other_formats(ctx):
     A = CASES["rand_small"]
     dA = cs.DeviceMatrix(ctx, cs.coo2ell(A.copy()))
