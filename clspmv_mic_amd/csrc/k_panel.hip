@@ -15,7 +15,7 @@
 // broadcasts); each segment's sum goes to seg_val.  Pass 2: one work-group
 // per row block loads the block's segments (one contiguous range), puts
 // each at its slot of a row-major LDS layout, and each row adds its slots
-// in panel order to y.  The summation order is fixed by the encoding; no
+// in panel order to y (long rows spread over the work-group).  The summation order is fixed by the encoding; no
 // atomics.
 
 #include "device_common.hpp"
@@ -215,17 +215,19 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
 __global__ __launch_bounds__(launch::kBlockThreads) void panel_pass2_kernel(
     const int* __restrict__ blk_row, const int* __restrict__ blk_seg,
     const int* __restrict__ row_slot, const unsigned short* __restrict__ seg_dst,
-    const float* __restrict__ seg_val, float* __restrict__ y) {
+    const float* __restrict__ seg_val, const int* __restrict__ blk_long,
+    const int4* __restrict__ longs, float* __restrict__ y) {
     constexpr int NT = launch::kBlockThreads, K = launch::kBlockItems;
     __shared__ float slot[launch::kBlockSegsMax];
-    constexpr int R = launch::kBlockRowsMax / NT;  // rows per thread at most
+    constexpr int R = launch::kBlockRowsMax / NT;  // rows (and long rows) per thread at most
     static_assert(R * NT == launch::kBlockRowsMax, "rows of a block must split evenly over the threads");
     const int b = blockIdx.x, tid = threadIdx.x;
     const int row0 = blk_row[b], nrows = blk_row[b + 1] - row0;
     const int j0 = blk_seg[b], j1 = blk_seg[b + 1];
+    const int l0 = blk_long[b], nlong = blk_long[b + 1] - l0;
     // Every load of the block -- its segments, the slot range and y of each
-    // of this thread's rows -- is issued before the first is used, so the
-    // work-group waits for one memory latency.
+    // of this thread's rows, its long rows -- is issued before the first is
+    // used, so the work-group waits for about one memory latency.
     unsigned short d[K];
     float v[K];
 #pragma unroll
@@ -245,28 +247,39 @@ __global__ __launch_bounds__(launch::kBlockThreads) void panel_pass2_kernel(
         s1[k] = live ? rs[i + 1] : 0;
         yv[k] = live ? y[row0 + i] : 0.0f;
     }
+    // The first NT long rows (a block rarely has more) are prefetched too.
+    int4 lr = tid < nlong ? longs[l0 + tid] : int4{0, 0, 0, 0};
+    float ly = tid < nlong ? y[row0 + lr.x] : 0.0f;
 #pragma unroll
     for (int k = 0; k < K; ++k)
         if (j0 + tid + k * NT < j1) slot[d[k]] = v[k];
     __syncthreads();
-    // Each row adds its slots in order (panel order; a run cut at a
-    // sub-chunk boundary keeps its pieces in order).  A slot holds at most
-    // one sub-chunk (kPanelSub entries), so every row longer than
-    // kLongRowF64 has more than kLongRowF64 / kPanelSub slots: those rows
-    // add their slots in FP64 and round into y once.
+    // Rows of at most kLongSlots slots: each thread adds its rows' slots in
+    // order (panel order; a run cut at a sub-chunk boundary keeps its pieces
+    // in order), in FP32.
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * NT;
         const int s = s0[k], e = s1[k];
-        if (i >= nrows || s == e) continue;
-        if (e - s > launch::kLongRowF64 / launch::kPanelSub) {
-            double acc = slot[s];
-            for (int q = s + 1; q < e; ++q) acc += slot[q];
-            y[row0 + i] = (float)((double)yv[k] + acc);
-        } else {
-            float acc = slot[s];
-            for (int q = s + 1; q < e; ++q) acc += slot[q];
-            y[row0 + i] = yv[k] + acc;
+        if (i >= nrows || s == e || e - s > launch::kLongSlots) continue;
+        float acc = slot[s];
+        for (int q = s + 1; q < e; ++q) acc += slot[q];
+        y[row0 + i] = yv[k] + acc;
+    }
+    // Longer rows -- every row longer than kLongRowF64 among them -- spread
+    // one per thread over the whole work-group (so the few long rows of a
+    // block do not hold up every wave), summed in FP64 in slot order and
+    // rounded into y once.
+    for (int base = 0; base < nlong; base += NT) {  // uniform
+        if (base > 0) {
+            const int l = base + tid;
+            lr = l < nlong ? longs[l0 + l] : int4{0, 0, 0, 0};
+            ly = l < nlong ? y[row0 + lr.x] : 0.0f;
+        }
+        if (base + tid < nlong) {
+            double acc = slot[lr.y];
+            for (int q = lr.y + 1; q < lr.z; ++q) acc += slot[q];
+            y[row0 + lr.x] = (float)((double)ly + acc);
         }
     }
 }
@@ -294,7 +307,8 @@ hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
                       const unsigned short* pcol, int dtype, const void* pval,
                       const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
                       float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
-                      const int* row_slot, const unsigned short* seg_dst, float* y) {
+                      const int* row_slot, const unsigned short* seg_dst, const int* blk_long,
+                      const int* longs, float* y) {
     if (nwg <= 0 || nblocks <= 0) return hipSuccess;
     if (pw <= 0 || pw > kPanelMax) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * ((size_t)pw + 1);  // x panel + the padding entries' 0
@@ -320,7 +334,7 @@ hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(panel_pass2_kernel, dim3(nblocks), dim3(kBlockThreads), 0, s, blk_row, blk_seg,
-                       row_slot, seg_dst, seg_val, y);
+                       row_slot, seg_dst, seg_val, blk_long, reinterpret_cast<const int4*>(longs), y);
     return hipGetLastError();
 }
 

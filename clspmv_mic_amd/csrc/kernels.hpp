@@ -56,16 +56,22 @@ constexpr int kPanelScPerWg = 512;    // most sub-chunks of one pass-1 work-grou
 constexpr int kPanelsMax = 2048;      // most panels of one matrix (bounds the encoder's tables)
 constexpr int kBlockRowsMax = 4096;   // rows of a pass-2 row block
 constexpr int kBlockThreads = 1024;   // pass-2 work-group size
-constexpr int kBlockItems = 8;        // segments per pass-2 thread
+constexpr int kBlockItems = 16;       // segments per pass-2 thread
 constexpr int kBlockSegsMax = kBlockThreads * kBlockItems;  // < 65536 (16-bit slots)
 // Raises pass 1's dynamic LDS limit on the current device (call before the
 // first launch, outside graph capture).
 hipError_t panel_prepare();
+// Pass 2 spreads the rows with more than kLongSlots slots (blk_long / longs
+// list them per block) over its threads and sums them in FP64; a slot holds
+// at most one sub-chunk, so every row longer than kLongRowF64 is among them.
+constexpr int kLongSlots = 8;
+static_assert(kLongSlots <= kLongRowF64 / kPanelSub, "every row longer than kLongRowF64 must be a long row");
 hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
                       const unsigned short* pcol, int dtype, const void* pval,
                       const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
                       float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
-                      const int* row_slot, const unsigned short* seg_dst, float* y);
+                      const int* row_slot, const unsigned short* seg_dst, const int* blk_long,
+                      const int* longs, float* y);
 
 // ---- ELL / SELL --------------------------------------------------------------
 hipError_t ell(hipStream_t s, int rows, int hal, int ellnum, int rows_per_lane,

@@ -216,6 +216,24 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
         for (int j = j0; j < j1; ++j) seg_dst[j] = (unsigned short)cur[(seg_row[j] & 0x3FFFFFFF) - r0]++;
     }
     blk_seg[nb] = (int)nseg;
+    // Rows with more than kLongSlots slots, per block: pass 2 spreads them
+    // over its threads and sums them in FP64 ({row in block, first slot, end
+    // slot, 0}).
+    std::vector<int> blk_long(nb + 1, 0);
+    std::vector<int> longs;
+    for (int b = 0; b < nb; ++b) {
+        const int r0 = blk_row[b], nr = blk_row[b + 1] - r0;
+        const int* slot = &row_slot[(size_t)r0 + b];
+        blk_long[b] = (int)(longs.size() / 4);
+        for (int i = 0; i < nr; ++i)
+            if (slot[i + 1] - slot[i] > launch::kLongSlots) {
+                longs.push_back(i);
+                longs.push_back(slot[i]);
+                longs.push_back(slot[i + 1]);
+                longs.push_back(0);
+            }
+    }
+    blk_long[nb] = (int)(longs.size() / 4);
 
     // What the kernels rely on: every sub-chunk opens a segment, its
     // segments' slots exist, and its lanes read whole 4-entry groups; each
@@ -288,6 +306,8 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
     if (!st) st = upload_array(d, blk_row.data(), blk_row.size(), &d->pblk_row);
     if (!st) st = upload_array(d, blk_seg.data(), blk_seg.size(), &d->pblk_seg);
     if (!st) st = upload_array(d, row_slot.data(), row_slot.size(), &d->prow_slot);
+    if (!st) st = upload_array(d, blk_long.data(), blk_long.size(), &d->pblk_long);
+    if (!st) st = upload_array(d, longs.data(), longs.size(), &d->plong);
     if (st) return st;
     d->use_panel = true;
     return CLSPMV_OK;
@@ -307,6 +327,8 @@ void drop_encoding(clspmv_dmat* d, bool panel) {
         release_array(d, &d->pblk_row);
         release_array(d, &d->pblk_seg);
         release_array(d, &d->prow_slot);
+        release_array(d, &d->pblk_long);
+        release_array(d, &d->plong);
         d->pw = 0;
         d->use_panel = false;
         return;

@@ -111,7 +111,8 @@ int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
         e = launch::panel_spmv(s, A->pnwg, A->pwg, A->pw, I.cols, A->pcol, A->dtype,
                                A->dtype ? (const void*)A->plpval : (const void*)A->pval, A->psc_seg,
                                A->psc_start, x, xa,
-                               A->pseg, A->pnb, A->pblk_row, A->pblk_seg, A->prow_slot, A->pseg_dst, y);
+                               A->pseg, A->pnb, A->pblk_row, A->pblk_seg, A->prow_slot, A->pseg_dst,
+                               A->pblk_long, A->plong, y);
         if (e != hipSuccess)
             return fail(CLSPMV_ERR_HIP, "panel launch failed (format %d): %s", I.format,
                         hipGetErrorString(e));

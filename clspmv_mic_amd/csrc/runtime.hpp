@@ -57,6 +57,8 @@ struct clspmv_dmat {
     int* pblk_row = nullptr;   // first row of each row block (+ end)
     int* pblk_seg = nullptr;   // first segment of each row block (+ end)
     int* prow_slot = nullptr;  // per block: first slot of each row (+ end)
+    int* pblk_long = nullptr;  // per block: first of its long rows in plong (+ end)
+    int* plong = nullptr;      // long rows: {row in block, first slot, end slot, 0}
     bool use_panel = false;
     // Serving: per k, 1 when one multi-vector pass measured faster than k
     // SpMVs, 0 when not, -1 not measured yet.  Kept with the matrix so that
