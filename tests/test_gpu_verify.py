@@ -204,7 +204,8 @@ def test_long_rows_fp64_csr(ctx, oracle, dtype, k):
 def test_long_rows_fp64_panel(ctx, oracle, fmt, dtype):
     """Panel encoding: a row's segment sums (each at most 256 entries, a
     scan of depth <= 10 in FP32) are added in FP64 for rows with more than
-    16 segments, so long rows stay within 12 U whatever their length."""
+    8 segments (every row over 4096 entries among them), so long rows stay
+    within 12 U whatever their length."""
     A = LONG
     vals = _stored(A.val, dtype)
     owned = cs.coo2csr(A.copy()) if fmt == "csr" else A.sorted()
