@@ -52,7 +52,9 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
 // A group of four lanes owns one block row and every right-hand side (in
 // quads of four): each A block is loaded once and multiplied by NQ tiles of
 // X, so A is read once per SpMM whatever k is.  Lane t of a group loads row
-// t of the block and, per quad q, column 4q + t of X.
+// t of the block and, per quad q, column 4q + t of X.  A wave holds a slice
+// of 16 block rows whose step-j blocks are stored together: its A load of a
+// step is one 512-byte run and its column load one 64-byte run.
 // LONG (matrices with a block row longer than kLongRowF64 / 4 blocks): the
 // FP32 MFMA accumulators are added into FP64 ones every kSpmmChunk blocks,
 // so a row's rounding error grows with the chunk, not the row length, and
@@ -62,33 +64,32 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int brows, int rows, int cols, int k, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
     const unsigned short* __restrict__ X, int ldx, float* __restrict__ Y, int ldy) {
+    static_assert(launch::kSpmmSlice * 4 == kWave, "a wave holds one slice, four lanes per block row");
     const int lane = threadIdx.x & (kWave - 1);
-    const int br = (int)(((long long)blockIdx.x * kBlock + threadIdx.x) / 4);
-    const int t = lane & 3;
+    const int t = lane & 3, g = lane >> 2;
+    // One wave per slice, in XCD-aware order: neighbouring block rows, which
+    // share X rows, run on one XCD and share its L2.
+    const int sl = (int)(((long long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) / kWave);
+    if (sl >= (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice) return;  // the whole wave
+    const int br = sl * launch::kSpmmSlice + g;
     const bool active = br < brows;
-    int j0 = 0, n = 0;
-    if (active) {
-        j0 = rp[br];
-        n = rp[br + 1] - j0;
-    }
-    // The instruction is wave-wide: iterate to the longest block row of the
-    // wave, feeding zeros past a group's own blocks.
-    int nmax = n;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) nmax = max(nmax, __shfl_xor(nmax, m, kWave));
+    // The instruction is wave-wide: every row of the slice steps through the
+    // slice's width, padding slots feeding zeros.
+    const int base = rp[sl];
+    const int nmax = (rp[sl + 1] - base) / launch::kSpmmSlice;
     f32x4 acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // Operands of block `it` (zeros past the group's own blocks).
+    // Operands of step `it` (zeros for padding slots and past the width).
     auto fetch = [&](int it, s16x4& a, s16x4* b) {
         a = s16x4{0, 0, 0, 0};
-        int c0 = 0;
-        const bool has = it < n;
-        if (has) {
-            const int j = j0 + it;
-            a = *reinterpret_cast<const s16x4*>(val + (size_t)j * 16 + t * 4);
+        int c0 = -1;
+        if (it < nmax) {
+            const size_t j = (size_t)base + (size_t)it * launch::kSpmmSlice + g;
+            a = *reinterpret_cast<const s16x4*>(val + j * 16 + t * 4);  // padding holds zeros
             c0 = bcol[j];
         }
+        const bool has = c0 >= 0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int xc = q * 4 + t;
@@ -162,8 +163,8 @@ template <bool BF16, int NQ>
 hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                      const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
                      float* Y, int ldy) {
-    const long long threads = (long long)brows * 4;
-    const long long blocks = (threads + kBlock - 1) / kBlock;
+    const long long slices = (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice;
+    const long long blocks = (slices * kWave + kBlock - 1) / kBlock;
     if (long_rows)
         hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, true>), dim3((unsigned)blocks), dim3(kBlock), 0, s, brows,
                            rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);

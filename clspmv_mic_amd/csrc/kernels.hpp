@@ -116,14 +116,18 @@ hipError_t csr_multi(hipStream_t s, int nblocks, const int* blocks, const int* r
                      int dtype, const void* val, int k, const float* X, float* Y, int ldy);
 
 // ---- mixed-precision 4x4-block SpMM (MFMA) -------------------------------------
-// Y[:, 0:k] += A X[:, 0:k]; A: block rows rp/bcol (element columns, multiples
-// of 4) with 16 BF16 / FP16 values per block, row-major; X: 16-bit, column-
-// major (ldx); Y: FP32 column-major (ldy).  ldx, ldy multiples of 4 and X, Y
-// 16-byte aligned.  long_rows: some block row holds more than kLongRowF64 / 4
+// Y[:, 0:k] += A X[:, 0:k]; A in slices of kSpmmSlice block rows: sptr[s] is
+// slice s's first slot (its width is (sptr[s+1] - sptr[s]) / kSpmmSlice),
+// slot sptr[s] + j * kSpmmSlice + g holds block j of row s * kSpmmSlice + g:
+// its element column in scol (a multiple of 4, -1 for padding) and its 16
+// BF16 / FP16 values, row-major, in val.  X: 16-bit, column-major (ldx);
+// Y: FP32 column-major (ldy).  ldx, ldy multiples of 4 and X, Y 16-byte
+// aligned.  long_rows: some block row holds more than kLongRowF64 / 4
 // blocks; the MFMA sums are then carried in FP64 every kSpmmChunk blocks.
 constexpr int kSpmmChunk = 16;
+constexpr int kSpmmSlice = 16;  // block rows per slice: one wave, 4 lanes per block row
 hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows, int cols, int k,
-                  const int* rp, const int* bcol, const unsigned short* val,
+                  const int* sptr, const int* scol, const unsigned short* val,
                   const unsigned short* X, int ldx, float* Y, int ldy);
 
 // ---- bandwidth probe ---------------------------------------------------------

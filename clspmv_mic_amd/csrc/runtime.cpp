@@ -764,13 +764,35 @@ int clspmv_upload_bcsr_lowp(clspmv_ctx* ctx, const clspmv_bcsr_matrix* m, int dt
         if (c < 0 || c >= cols || c % 4)
             UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "BCSR block %d: column %d", j, c));
     }
-    // Element (h, w) of block j sits at data[j + (4h + w) * aligned]
-    // (include/matrix_storage.h:1719-1735); the device keeps each block's 16
-    // values together, row by row.
-    std::vector<float> blocks((size_t)nb * 16);
-    for (int j = 0; j < nb; ++j)
-        for (int e = 0; e < 16; ++e)
-            blocks[(size_t)j * 16 + e] = m->bcsr_data[j + (size_t)e * m->bcsr_aligned_size];
+    // Device layout: slices of kSpmmSlice block rows (one wave's), each
+    // padded to its longest block row; block j of a slice's rows are stored
+    // together, so a wave's loads of one step are contiguous.  Slot
+    // (slice s, step j, row g) holds the block's column (-1 for padding) and
+    // its 16 values row by row; element (h, w) of BCSR block b sits at
+    // data[b + (4h + w) * aligned] (include/matrix_storage.h:1719-1735).
+    constexpr int S = launch::kSpmmSlice;
+    const int nsl = (nbr + S - 1) / S;
+    std::vector<int> sptr((size_t)nsl + 1, 0);
+    for (int sl = 0; sl < nsl; ++sl) {
+        int w = 0;
+        for (int r = sl * S; r < std::min(nbr, (sl + 1) * S); ++r)
+            w = std::max(w, m->bcsr_row_ptr[r + 1] - m->bcsr_row_ptr[r]);
+        if ((long long)sptr[sl] + (long long)w * S > INT_MAX / 16)
+            UPLOAD_CHECK(fail(CLSPMV_ERR_TOO_LARGE, "block SpMM layout exceeds 32-bit offsets"));
+        sptr[sl + 1] = sptr[sl] + w * S;
+    }
+    const size_t slots = (size_t)sptr[nsl];
+    std::vector<int> scol(slots, -1);
+    std::vector<float> blocks(slots * 16, 0.0f);
+    for (int r = 0; r < nbr; ++r) {
+        const int sl = r / S, g = r % S;
+        for (int j = 0; j < m->bcsr_row_ptr[r + 1] - m->bcsr_row_ptr[r]; ++j) {
+            const int b = m->bcsr_row_ptr[r] + j;
+            const size_t at = (size_t)sptr[sl] + (size_t)j * S + g;
+            scol[at] = m->bcsr_col_id[b];
+            for (int e = 0; e < 16; ++e) blocks[at * 16 + e] = m->bcsr_data[b + (size_t)e * m->bcsr_aligned_size];
+        }
+    }
     std::vector<uint16_t> lp(blocks.size());
     UPLOAD_CHECK(clspmv_convert_from_f32(blocks.data(), lp.data(), lp.size(), dtype));
     d->dtype = dtype;
@@ -778,9 +800,9 @@ int clspmv_upload_bcsr_lowp(clspmv_ctx* ctx, const clspmv_bcsr_matrix* m, int dt
     for (int r = 0; r < nbr && !d->long_brows; ++r)
         d->long_brows = (long long)(m->bcsr_row_ptr[r + 1] - m->bcsr_row_ptr[r]) * 4 > launch::kLongRowF64;
     d->info.bheight = d->info.bwidth = 4;
-    d->info.stored = (long long)nb * 16;
-    UPLOAD_CHECK(upload_array(d, m->bcsr_row_ptr, (size_t)nbr + 1, &d->rp));
-    UPLOAD_CHECK(upload_array(d, m->bcsr_col_id, (size_t)nb, &d->col));
+    d->info.stored = (long long)slots * 16;
+    UPLOAD_CHECK(upload_array(d, sptr.data(), sptr.size(), &d->rp));
+    UPLOAD_CHECK(upload_array(d, scol.data(), scol.size(), &d->col));
     UPLOAD_CHECK(upload_array(d, lp.data(), lp.size(), &d->lpval));
     *out = d;
     return CLSPMV_OK;
