@@ -389,9 +389,15 @@ int tune_parts(clspmv_cocktail_kernels* k, int ntimes, float* y) {
         auto trial = [&](int m, int l, bool panel) -> int {
             A->lanes = l;
             A->use_panel = panel;
-            double ms = 0.0;
-            int st = time_runs(k, ntimes, [&](hipStream_t s) { return dmat_launch(s, A, m, g->vec, y); }, &ms);
-            if (st) return st;
+            // Best of three batches: one batch's average can carry a stall
+            // that flips the choice between kernels a few microseconds apart.
+            double ms = 1e30;
+            for (int rep = 0; rep < 3; ++rep) {
+                double t = 0.0;
+                int st = time_runs(k, ntimes, [&](hipStream_t s) { return dmat_launch(s, A, m, g->vec, y); }, &t);
+                if (st) return st;
+                ms = std::min(ms, t);
+            }
             if (ms < best) {
                 best = ms;
                 best_m = m;
