@@ -57,12 +57,6 @@ __device__ __forceinline__ void seg_step(float& s, int& g) {
     g |= tg;
 }
 
-// Number of set bits of `mask` below this lane.
-__device__ __forceinline__ int lanes_below(unsigned long long mask) {
-    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
-                                          __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-}
-
 struct SubChunk {
     u32x2 c;   // four 16-bit entry words
     f32x4 v;   // four values
@@ -84,16 +78,23 @@ __device__ __forceinline__ void seg_scan_write(const SubChunk& q, const float* _
         p[k] = v[k] * xs[min(w[k] & 0x7FFFu, pad)];
     }
     if (lane == 0) f[0] = 1;  // every sub-chunk opens a segment
-    // Segment numbering: flags of earlier lanes, then this lane's own.
-    int before = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) before += lanes_below(__ballot(f[k]));
     // Running sums inside the lane, restarted at each flag.
     float r[4];
     r[0] = p[0];
 #pragma unroll
     for (int k = 1; k < 4; ++k) r[k] = f[k] ? p[k] : r[k - 1] + p[k];
     const int c0 = f[0], c1 = c0 + f[1], c2 = c1 + f[2], nf = c2 + f[3];
+    // Segment numbering: the flags of earlier lanes (an exclusive scan of
+    // nf over the wave: DPP row shifts, then the row broadcasts), then this
+    // lane's own.
+    int incl = nf;
+    incl += dpp_int<kRowShr1, 0xF>(incl);
+    incl += dpp_int<kRowShr2, 0xF>(incl);
+    incl += dpp_int<kRowShr4, 0xF>(incl);
+    incl += dpp_int<kRowShr8, 0xF>(incl);
+    incl += dpp_int<kRowBcast15, 0xA>(incl);
+    incl += dpp_int<kRowBcast31, 0xC>(incl);
+    const int before = incl - nf;
     // Segments that open and close inside the lane.
     if (f[1] && c0) out[before + c0 - 1] = r[0];
     if (f[2] && c1) out[before + c1 - 1] = r[1];
