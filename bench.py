@@ -394,6 +394,12 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes": dom["bytes"],
             "kernel_ms": round(dom["ms"], 6),
+            # The bytes the part's format must move (DIA and block formats
+            # read no per-nonzero column index, so the algorithmic count
+            # above can exceed what they move and frac can pass 1).
+            "format_bytes": dom["format_bytes"],
+            "format_frac": round(dom["format_bytes"] / (dom["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4)
+            if dom["ms"] > 0 else 0.0,
         },
         "parts": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in p.items()} for p in parts],
         "graph_ms": round(graph_ms, 6),

@@ -389,12 +389,22 @@ int tune_parts(clspmv_cocktail_kernels* k, int ntimes, float* y) {
         auto trial = [&](int m, int l, bool panel) -> int {
             A->lanes = l;
             A->use_panel = panel;
-            // Best of three batches: one batch's average can carry a stall
-            // that flips the choice between kernels a few microseconds apart.
+            g->methods[i] = m;
+            // The whole pass is timed, not the part alone: a part run by
+            // itself keeps x and y in cache between launches, which the
+            // other parts' streams evict in a real pass.  Best of three
+            // batches: one batch's average can carry a stall that flips the
+            // choice between kernels a few microseconds apart.
             double ms = 1e30;
             for (int rep = 0; rep < 3; ++rep) {
                 double t = 0.0;
-                int st = time_runs(k, ntimes, [&](hipStream_t s) { return dmat_launch(s, A, m, g->vec, y); }, &t);
+                int st = time_runs(k, ntimes, [&](hipStream_t s) {
+                    for (size_t p = 0; p < g->parts.size(); ++p) {
+                        const int r = dmat_launch(s, g->parts[p], g->methods[p], g->vec, y);
+                        if (r) return r;
+                    }
+                    return (int)CLSPMV_OK;
+                }, &t);
                 if (st) return st;
                 ms = std::min(ms, t);
             }
@@ -418,6 +428,7 @@ int tune_parts(clspmv_cocktail_kernels* k, int ntimes, float* y) {
         if (st) {
             A->lanes = upload_lanes;
             A->use_panel = A->pw && !A->plain;
+            g->methods[i] = best_m;
             return st;
         }
         g->methods[i] = best_m;

@@ -158,8 +158,9 @@ int clspmv_cocktail_profile(clspmv_cocktail_kernels* kernels, int ntimes,
  * runtime variant probe (init_mat_kernels with use_tex,
  * src/spmv_cocktail.cpp:1590): every method a part's shape allows (and,
  * for CSR method 0 and BCSR, the neighbouring lane counts) is timed with
- * HIP events over ntimes launches (best of three such batches) and the
- * fastest kept (CSR and COO parts
+ * HIP events over ntimes whole passes (best of three such batches; the
+ * other parts run their current choices) and the fastest kept (CSR and COO
+ * parts
  * are uploaded in both their own and the x-panel encoding, and the losing
  * encoding is freed); then the captured
  * graph is re-built and compared with plain stream launches of the same
