@@ -249,6 +249,25 @@ def run_serve(args):
     # Poisson arrivals at a fraction of capacity.
     rate = args.load * cap
     lat, wall = server.replay(trace(rate), mode=0)
+    # The highest arrival rate whose p99 latency stays within the SLO:
+    # bisection over the load (fraction of capacity) on shorter traces.
+    ns = min(n, args.slo_requests)
+
+    def slo_load(mode, full):
+        lo, hi = 0.0, 1.0
+        for _ in range(args.slo_steps):
+            mid = 0.5 * (lo + hi)
+            l, _ = server.replay(trace(mid * full)[:ns], mode=mode)
+            if float(np.percentile(l, 99)) * 1e3 <= args.slo_ms:
+                lo = mid
+            else:
+                hi = mid
+        return lo
+
+    slo_load_served = slo_load(0, cap)
+    slo_rate = slo_load_served * cap
+    # The same for the plain launches, whose requests wait for each other.
+    slo_rate_plain = slo_load(1, plain) * plain
     server.close()
     served = n / wall
     flops = 2.0 * float(np.sum(nnz_of[mat] * ks))
@@ -266,6 +285,11 @@ def run_serve(args):
         "capacity_requests_per_s": round(cap, 1),
         "plain_per_request_requests_per_s": round(plain, 1),
         "speedup_vs_plain": round(cap / plain, 2),
+        "slo": {"p99_ms": args.slo_ms, "requests_per_s": round(slo_rate, 1),
+                "load": round(slo_load_served, 4),
+                "plain_requests_per_s": round(slo_rate_plain, 1),
+                "speedup_vs_plain": round(slo_rate / slo_rate_plain, 2) if slo_rate_plain > 0 else None,
+                "requests_per_probe": ns},
         "plain_latency_ms_p50": round(float(np.percentile(latb, 50)) * 1e3, 3),
         "gflops_served": round(flops / wall / 1e9, 2),
         "bf16_matrix_error_vs_fp64": {"matrices": len(errs16),
@@ -293,6 +317,9 @@ def main():
     ap.add_argument("--baseline-requests", type=int, default=2000, help="serve: plain-launch sample")
     ap.add_argument("--load", type=float, default=0.7, help="serve: Poisson rate / capacity")
     ap.add_argument("--streams", type=int, default=8, help="serve: HIP streams")
+    ap.add_argument("--slo-ms", type=float, default=2.0, help="serve: p99 latency target")
+    ap.add_argument("--slo-requests", type=int, default=4000, help="serve: requests per SLO probe")
+    ap.add_argument("--slo-steps", type=int, default=6, help="serve: bisection steps for the SLO rate")
     ap.add_argument("--tables", default=None, help="directory holding benchmark/*.ben")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the tables' methods and graph launches")
