@@ -66,7 +66,7 @@ def _run(ctx, oracle, A, dtype, k, pad=0):
 
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("dtype", [cs.DT_BF16, cs.DT_F16])
-@pytest.mark.parametrize("k", [1, 3, 4, 8, 16, 32])
+@pytest.mark.parametrize("k", [1, 3, 4, 8, 16, 32, 40])
 def test_spmm_matches_oracle(ctx, oracle, name, dtype, k):
     _run(ctx, oracle, CASES[name], dtype, k)
 
