@@ -71,18 +71,19 @@ __device__ __forceinline__ void seg_scan_write(const SubChunk& q, const float* _
     const unsigned w[4] = {q.c.x & 0xFFFFu, q.c.x >> 16, q.c.y & 0xFFFFu, q.c.y >> 16};
     const float v[4] = {q.v.x, q.v.y, q.v.z, q.v.w};
     int f[4];
-    float p[4];
+    float xv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         f[k] = (int)(w[k] >> 15);
-        p[k] = v[k] * xs[min(w[k] & 0x7FFFu, pad)];
+        xv[k] = xs[min(w[k] & 0x7FFFu, pad)];
     }
     if (lane == 0) f[0] = 1;  // every sub-chunk opens a segment
-    // Running sums inside the lane, restarted at each flag.
+    // Running sums inside the lane, restarted at each flag (one rounding
+    // per entry: the product is fused into the add).
     float r[4];
-    r[0] = p[0];
+    r[0] = v[0] * xv[0];
 #pragma unroll
-    for (int k = 1; k < 4; ++k) r[k] = f[k] ? p[k] : r[k - 1] + p[k];
+    for (int k = 1; k < 4; ++k) r[k] = fmaf(v[k], xv[k], f[k] ? 0.0f : r[k - 1]);
     const int c0 = f[0], c1 = c0 + f[1], c2 = c1 + f[2], nf = c2 + f[3];
     // Segment numbering: the flags of earlier lanes (an exclusive scan of
     // nf over the wave: DPP row shifts, then the row broadcasts), then this
@@ -206,7 +207,11 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int c = sc + u * nw;
-            if (c < sc1) seg_scan_write(cur[u], xs, (unsigned)pw, lane, seg_val + s_seg[c - sc0]);
+            // The segment base is wave-uniform: a scalar base lets every
+            // store take a 32-bit lane offset.
+            if (c < sc1)
+                seg_scan_write(cur[u], xs, (unsigned)pw, lane,
+                               seg_val + __builtin_amdgcn_readfirstlane(s_seg[c - sc0]));
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
