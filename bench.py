@@ -302,13 +302,65 @@ def run_serve(args):
     return 0
 
 
+def _driver_quiet(cs, name, coo, ntimes):
+    """run_driver with the driver's own progress lines (C stdout) sent to
+    stderr, so that stdout carries only the JSON lines."""
+    import ctypes
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        rep, _ = cs.run_driver(name, coo, ntimes=ntimes, want_y=False)
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+    return rep
+
+
+def run_sweep(args):
+    """Every per-format driver (spmv_<fmt>, include/spmv_<fmt>.h:7) on the
+    power-law 1M + band matrix: one JSON line per format with the best
+    variant's GFLOP/s, or the reason the format was refused."""
+    import numpy as np
+    import clspmv_mic_amd as cs
+    from clspmv_mic_amd import matrices
+    A = matrices.power_law(1_000_000, seed=2)
+    longest = int(np.bincount(A.row, minlength=A.height).max())
+    ell_entries = longest * ((A.height + 31) // 32 * 32)
+    for name in ["csr_scalar", "csr_vector", "coo", "sell", "ell", "dia", "bdia", "bcsr", "bell", "sbell"]:
+        out = {"metric": "SpMV GFLOP/s per format (best variant of the format's driver)",
+               "value": None, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": 3,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+               "data": "synthetic",
+               "config": {"workload": "power-law 1M rows, Zipf(1.8) in [1,5000] + band {0,+-1,+-1000}",
+                          "nnz": A.nnz, "format": name}}
+        t0 = time.time()
+        if name == "ell" and ell_entries > (1 << 28):
+            out["config"]["status"] = (f"refused: ELL pads every row to the longest ({longest}), "
+                                       f"{ell_entries} entries")
+        else:
+            try:
+                rep = _driver_quiet(cs, name, A.copy(), args.steps)
+                out["value"] = round(rep.best_gflops, 3)
+                out["ms_per_step"] = round(rep.best_ms, 6)
+                out["config"].update({"status": "ok", "variants": rep.variants, "best_method": rep.best_method,
+                                      "best_block": [rep.best_bheight, rep.best_bwidth]})
+            except cs.ClspmvError as e:
+                out["config"]["status"] = f"refused: {e}"
+        out["config"]["seconds"] = round(time.time() - t0, 1)
+        print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="powerlaw",
-                    choices=["powerlaw", "stencil27", "lap2d", "block4x4", "spmm", "serve"])
+                    choices=["powerlaw", "stencil27", "lap2d", "block4x4", "spmm", "serve", "sweep"])
     ap.add_argument("--dtype", default="bf16", choices=["bf16", "fp16"], help="spmm: value type")
     ap.add_argument("--k", type=int, nargs="+", default=[8, 16, 32], help="spmm: right-hand sides")
     ap.add_argument("--err-columns", type=int, default=4, help="spmm: columns checked against FP64")
@@ -334,6 +386,8 @@ def main():
         return run_spmm(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "sweep":
+        return run_sweep(args)
 
     import numpy as np
     import clspmv_mic_amd as cs
