@@ -384,6 +384,51 @@ def test_csr_16bit_both_encodings(ctx, oracle, dtype):
         assert err <= TOL, f"rhs {q}: {err:.3e} at row {worst}"
 
 
+def test_panel_block_with_many_long_rows(ctx, oracle):
+    """Pass-2 blocks holding more than 1024 rows of more than 8 slots (the
+    long-row list then takes more than one round of the work-group)."""
+    # Every row: one entry in each of 10 distinct panels (of 16), so 10 slots.
+    rng = np.random.default_rng(17)
+    n, pw = 700_000, 4096
+    r = np.repeat(np.arange(n, dtype=np.int64), 10)
+    j = np.tile(np.arange(10, dtype=np.int64), n)
+    cols = ((r + 3 * j) % 16) * pw + rng.integers(0, pw, size=r.size)
+    A = cs.Coo(r, cols, rng.standard_normal(r.size).astype(np.float32), n, 16 * pw).sorted()
+    dA = cs.DeviceMatrix(ctx, cs.coo2csr(A.copy()), panel=pw)
+    assert dA.info.panel_active == 1
+    assert n / dA.info.panel_blocks > 1024  # more than 1024 long rows per block
+    x = matrices.uniform_vector(A.width, seed=5)
+    y0 = matrices.uniform_vector(A.height, seed=6)
+    y = cs.DeviceVector(ctx, A.height, y0)
+    dA.spmv(cs.DeviceVector(ctx, A.width, x), y)
+    ctx.sync()
+    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+    err, worst = oracle.scaled_error(y.download(), y64, bound)
+    assert err <= TOL, f"{err:.3e} at row {worst}"
+
+
+def test_panel_row_too_long_falls_back(ctx, oracle):
+    """A row whose segments exceed one pass-2 block: an explicit panel width
+    is refused, the automatic choice keeps the plain arrays; both give the
+    oracle's result where they run."""
+    rng = np.random.default_rng(23)
+    n_long = 4_500_000
+    rows = np.concatenate([np.zeros(n_long, np.int64), rng.integers(1, 2000, size=20000)])
+    cols = rng.integers(0, 4_000_000, size=rows.size)
+    A = cs.Coo(rows, cols, rng.standard_normal(rows.size).astype(np.float32), 2000, 4_000_000).sorted()
+    with pytest.raises(cs.ClspmvError):
+        cs.DeviceMatrix(ctx, A, panel=32760)
+    dA = cs.DeviceMatrix(ctx, A, panel=cs.PANEL_AUTO)
+    assert dA.info.panel_active == 0 and dA.info.plain == 1
+    x = matrices.uniform_vector(A.width, seed=5)
+    y = cs.DeviceVector(ctx, A.height, np.zeros(A.height, np.float32))
+    dA.spmv(cs.DeviceVector(ctx, A.width, x), y)
+    ctx.sync()
+    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height)
+    err, worst = oracle.scaled_error(y.download(), y64, bound)
+    assert err <= TOL, f"{err:.3e} at row {worst}"
+
+
 def test_spmv_multi_refuses_other_formats(ctx):
     A = CASES["rand_small"]
     dA = cs.DeviceMatrix(ctx, cs.coo2ell(A.copy()))
