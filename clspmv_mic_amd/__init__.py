@@ -300,6 +300,8 @@ if hasattr(lib, "clspmv_init_mat_kernels"):
          P(C.c_double))
     _sig("clspmv_cocktail_tune", C.c_int, VP, C.c_int, P(C.c_int))
     _sig("clspmv_cocktail_set_graph", C.c_int, VP, C.c_int)
+    _sig("clspmv_cocktail_set_overlap", C.c_int, VP, C.c_int)
+    _sig("clspmv_cocktail_launch_modes", C.c_int, VP, P(C.c_int), P(C.c_int))
     _sig("clspmv_evaluate", C.c_int, P(CocktailMatrix), PF, PF, PF, P(EvalReport))
 
 
@@ -948,6 +950,18 @@ class CocktailRuntime:
     def set_graph(self, graph: bool):
         """One graph launch per pass (True) or plain kernel launches."""
         check(lib.clspmv_cocktail_set_graph(self.k, int(graph)), "cocktail_set_graph")
+
+    def set_overlap(self, overlap: bool):
+        """Run the x-panel parts' first pass on a second stream beside the
+        earlier parts (True, the default) or everything on one stream."""
+        check(lib.clspmv_cocktail_set_overlap(self.k, int(overlap)), "cocktail_set_overlap")
+
+    def launch_modes(self) -> dict:
+        """{"graph": bool, "overlap": bool}; overlap is True only when a pass
+        actually splits across the two streams."""
+        g, o = C.c_int(), C.c_int()
+        check(lib.clspmv_cocktail_launch_modes(self.k, C.byref(g), C.byref(o)), "cocktail_launch_modes")
+        return {"graph": bool(g.value), "overlap": bool(o.value)}
 
     def close(self):
         if self.k:

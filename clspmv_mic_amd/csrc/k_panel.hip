@@ -131,7 +131,11 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
     // from LDS instead of one dependent memory load per sub-chunk).
     __shared__ int s_start[launch::kPanelScPerWg + 1];
     __shared__ int s_seg[launch::kPanelScPerWg];
-    const int4 t = wg[blockIdx.x];
+    // The table lists a panel's work-groups together; in XCD-aware order
+    // they run on one XCD, whose L2 then fetches that panel of x once
+    // (block i goes to XCD i mod 8, so the plain order would have every
+    // XCD stage every panel).
+    const int4 t = wg[xcd_tile(blockIdx.x, gridDim.x)];
     const int panel = t.x, sc0 = t.y, sc1 = t.z;
     const int xb = panel * pw;
     const int w = min(pw, cols - xb);
@@ -309,13 +313,10 @@ hipError_t panel_prepare() {
     return e;
 }
 
-hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
-                      const unsigned short* pcol, int dtype, const void* pval,
-                      const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
-                      float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
-                      const int* row_slot, const unsigned short* seg_dst, const int* blk_long,
-                      const int* longs, float* y) {
-    if (nwg <= 0 || nblocks <= 0) return hipSuccess;
+hipError_t panel_pass1(hipStream_t s, int nwg, const int* wg, int pw, int cols, const unsigned short* pcol,
+                       int dtype, const void* pval, const int* sc_seg, const int* sc_start, const float* x,
+                       bool x_aligned, float* seg_val) {
+    if (nwg <= 0) return hipSuccess;
     if (pw <= 0 || pw > kPanelMax) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * ((size_t)pw + 1);  // x panel + the padding entries' 0
     const int4* wg4 = reinterpret_cast<const int4*>(wg);
@@ -337,8 +338,13 @@ hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
         default:
             return hipErrorInvalidValue;
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t panel_pass2(hipStream_t s, const float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
+                       const int* row_slot, const unsigned short* seg_dst, const int* blk_long,
+                       const int* longs, float* y) {
+    if (nblocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(panel_pass2_kernel, dim3(nblocks), dim3(kBlockThreads), 0, s, blk_row, blk_seg,
                        row_slot, seg_dst, seg_val, blk_long, reinterpret_cast<const int4*>(longs), y);
     return hipGetLastError();

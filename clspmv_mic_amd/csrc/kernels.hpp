@@ -66,12 +66,16 @@ hipError_t panel_prepare();
 // at most one sub-chunk, so every row longer than kLongRowF64 is among them.
 constexpr int kLongSlots = 8;
 static_assert(kLongSlots <= kLongRowF64 / kPanelSub, "every row longer than kLongRowF64 must be a long row");
-hipError_t panel_spmv(hipStream_t s, int nwg, const int* wg, int pw, int cols,
-                      const unsigned short* pcol, int dtype, const void* pval,
-                      const int* sc_seg, const int* sc_start, const float* x, bool x_aligned,
-                      float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
-                      const int* row_slot, const unsigned short* seg_dst, const int* blk_long,
-                      const int* longs, float* y);
+// Pass 1 reads x and writes the segment sums seg_val; pass 2 reads seg_val
+// and adds the row sums into y.  Pass 1 touches nothing pass 2 of another
+// matrix or any other kernel writes, so it may run on a second stream while
+// earlier kernels update y, as long as pass 2 waits for it.
+hipError_t panel_pass1(hipStream_t s, int nwg, const int* wg, int pw, int cols, const unsigned short* pcol,
+                       int dtype, const void* pval, const int* sc_seg, const int* sc_start, const float* x,
+                       bool x_aligned, float* seg_val);
+hipError_t panel_pass2(hipStream_t s, const float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
+                       const int* row_slot, const unsigned short* seg_dst, const int* blk_long,
+                       const int* longs, float* y);
 
 // ---- ELL / SELL --------------------------------------------------------------
 hipError_t ell(hipStream_t s, int rows, int hal, int ellnum, int rows_per_lane,

@@ -96,28 +96,57 @@ int auto_method(const clspmv_dmat* A) {
     }
 }
 
+bool dmat_has_prepass(const clspmv_dmat* A) {
+    const clspmv_dmat_info& I = A->info;
+    return (I.format == CLSPMV_FMT_CSR || I.format == CLSPMV_FMT_COO) && A->use_panel && I.rows > 0 &&
+           I.cols > 0 && I.nnz > 0;
+}
+
+int dmat_prepass(hipStream_t s, const clspmv_dmat* A, const float* x) {
+    if (!dmat_has_prepass(A)) return CLSPMV_OK;
+    const hipError_t e = launch::panel_pass1(s, A->pnwg, A->pwg, A->pw, A->info.cols, A->pcol, A->dtype,
+                                             A->dtype ? (const void*)A->plpval : (const void*)A->pval,
+                                             A->psc_seg, A->psc_start, x, aligned16(x), A->pseg);
+    if (e != hipSuccess)
+        return fail(CLSPMV_ERR_HIP, "panel pass 1 failed (format %d): %s", A->info.format, hipGetErrorString(e));
+    return CLSPMV_OK;
+}
+
+namespace {
+int resolve_method(const clspmv_dmat* A, int* method) {
+    if (*method == CLSPMV_METHOD_AUTO) *method = auto_method(A);
+    if (*method < 0 || *method >= clspmv_method_count(A->info.format))
+        return fail(CLSPMV_ERR_UNSUPPORTED, "method %d not available for format %d", *method,
+                    A->info.format);
+    return CLSPMV_OK;
+}
+}  // namespace
+
 int dmat_launch(hipStream_t s, const clspmv_dmat* A, int method,
                 const float* x, float* y) {
+    int st = resolve_method(A, &method);
+    if (!st) st = dmat_prepass(s, A, x);
+    return st ? st : dmat_launch_rest(s, A, method, x, y);
+}
+
+int dmat_launch_rest(hipStream_t s, const clspmv_dmat* A, int method,
+                     const float* x, float* y) {
     const clspmv_dmat_info& I = A->info;
-    if (method == CLSPMV_METHOD_AUTO) method = auto_method(A);
-    if (method < 0 || method >= clspmv_method_count(I.format))
-        return fail(CLSPMV_ERR_UNSUPPORTED, "method %d not available for format %d",
-                    method, I.format);
+    const int st = resolve_method(A, &method);
+    if (st) return st;
     if (I.rows == 0 || I.cols == 0 || I.nnz == 0) return CLSPMV_OK;
     hipError_t e = hipSuccess;
-    const bool xa = aligned16(x);
-    if ((I.format == CLSPMV_FMT_CSR || I.format == CLSPMV_FMT_COO) && A->use_panel) {
-        // The panel encoding serves every method of the format.
-        e = launch::panel_spmv(s, A->pnwg, A->pwg, A->pw, I.cols, A->pcol, A->dtype,
-                               A->dtype ? (const void*)A->plpval : (const void*)A->pval, A->psc_seg,
-                               A->psc_start, x, xa,
-                               A->pseg, A->pnb, A->pblk_row, A->pblk_seg, A->prow_slot, A->pseg_dst,
-                               A->pblk_long, A->plong, y);
+    if (dmat_has_prepass(A)) {
+        // The panel encoding serves every method of the format; pass 1 has
+        // filled the segment sums.
+        e = launch::panel_pass2(s, A->pseg, A->pnb, A->pblk_row, A->pblk_seg, A->prow_slot, A->pseg_dst,
+                                A->pblk_long, A->plong, y);
         if (e != hipSuccess)
-            return fail(CLSPMV_ERR_HIP, "panel launch failed (format %d): %s", I.format,
+            return fail(CLSPMV_ERR_HIP, "panel pass 2 failed (format %d): %s", I.format,
                         hipGetErrorString(e));
         return CLSPMV_OK;
     }
+    const bool xa = aligned16(x);
     switch (I.format) {
         case CLSPMV_FMT_CSR:
             if (A->dtype) {  // 16-bit values: the multi-vector kernel with k = 1

@@ -142,6 +142,16 @@ int use_ctx_device(clspmv_ctx* ctx);
 int dmat_launch(hipStream_t stream, const clspmv_dmat* A, int method,
                 const float* x, float* y);
 
+// The same launch in two phases, for callers that overlap one matrix's
+// x-only work with other kernels: dmat_prepass issues the kernels that read
+// x and write only the matrix's own scratch (panel pass 1; nothing for
+// other encodings), dmat_launch_rest the ones that update y, which must be
+// ordered after the prepass.  dmat_launch = prepass then rest.
+bool dmat_has_prepass(const clspmv_dmat* A);
+int dmat_prepass(hipStream_t stream, const clspmv_dmat* A, const float* x);
+int dmat_launch_rest(hipStream_t stream, const clspmv_dmat* A, int method,
+                     const float* x, float* y);
+
 // Multi-vector SpMV (DIA, CSR with its own arrays): Y[:, q] += A X[:, q],
 // X row-major, Y column-major; no argument checks.
 bool has_multi(const clspmv_dmat* A);

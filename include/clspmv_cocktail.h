@@ -163,8 +163,10 @@ int clspmv_cocktail_profile(clspmv_cocktail_kernels* kernels, int ntimes,
  * parts
  * are uploaded in both their own and the x-panel encoding, and the losing
  * encoding is freed); then the captured
- * graph is re-built and compared with plain stream launches of the same
- * pass, and the faster launch mode is kept.  y is preserved.  Tuning picks
+ * graph is re-built, and graph launches and plain stream launches of the
+ * same pass are timed, each with and without the x-panel first passes on a
+ * second stream (clspmv_cocktail_set_overlap); the fastest of the four is
+ * kept.  y is preserved.  Tuning picks
  * by timing, so two processes may settle on different methods whose
  * results differ by rounding; untuned kernels follow the tables' choice.
  * `graph` (may be null) receives 1 when passes run as a graph launch. */
@@ -172,6 +174,18 @@ int clspmv_cocktail_tune(clspmv_cocktail_kernels* kernels, int ntimes, int* grap
 /* Sets the launch mode by hand: 1 = one graph launch per pass (the default
  * after init), 0 = plain launches of the parts' kernels. */
 int clspmv_cocktail_set_graph(clspmv_cocktail_kernels* kernels, int graph);
+/* Sets whether the x-panel parts' first pass (which reads only x) runs on
+ * a second stream beside the parts launched before them, the default after
+ * init; 0 = every kernel on one stream.  Either way each row of y receives
+ * the parts' contributions in the same order, so results are identical.
+ * clspmv_cocktail_tune times both and keeps the faster.  Re-builds the
+ * graph. */
+int clspmv_cocktail_set_overlap(clspmv_cocktail_kernels* kernels, int overlap);
+/* The launch modes in force: *graph = 1 for one graph launch per pass;
+ * *overlap = 1 when a pass actually runs a first pass on the second stream
+ * (overlap set, and an x-panel part follows another part).  Either pointer
+ * may be null. */
+int clspmv_cocktail_launch_modes(const clspmv_cocktail_kernels* kernels, int* graph, int* overlap);
 
 /* evaluate (src/spmv_cocktail.cpp:1431): runs one SpMV from res, compares
  * with coores (the caller's expected y, e.g. from a serial COO SpMV) using

@@ -125,3 +125,37 @@ def test_tuned_runtime_dyadic_exact(oracle, mname):
         rt.do_spmv(1)
         np.testing.assert_array_equal(rt.get_y(), y64.astype(np.float32), err_msg=f"graph={graph}")
     rt.close()
+
+
+def test_overlapped_first_pass_identical(oracle):
+    """A power-law matrix with a band partitions into DIA + an x-panel COO
+    part; the COO first pass runs on a second stream beside the DIA kernel.
+    The pass must give the same y, bit for bit, as the one-stream pass, in
+    both launch modes and over repeated passes, and stay within the
+    tolerance of the FP64 oracle."""
+    A = matrices.power_law(200000, seed=7)  # DIA (the band) + COO under the shipped tables
+    ck = cs.analyze_matrix(A.copy(), cs.TABLES_DIR)
+    summ = ck.summary()
+    assert summ.get("dia", 0) > 0 and summ.get("coo", 0) + summ.get("csr", 0) >= 65536, summ
+    x = matrices.uniform_vector(A.width, seed=31)
+    y0 = matrices.uniform_vector(A.height, seed=32)
+    rt = cs.CocktailRuntime(ck, x, y0)
+    assert rt.launch_modes()["overlap"], "an x-panel part after the DIA part should overlap"
+    outs = {}
+    for overlap in (True, False):
+        rt.set_overlap(overlap)
+        assert rt.launch_modes()["overlap"] == overlap
+        for graph in (True, False):
+            rt.set_graph(graph)
+            rt.set_y(y0)
+            rt.do_spmv(3)
+            outs[(overlap, graph)] = rt.get_y()
+    rt.close()
+    ref = outs[(False, False)]
+    for key, y in outs.items():
+        np.testing.assert_array_equal(y, ref, err_msg=f"overlap, graph = {key}")
+    # Three passes from y0: y0 + 3 A x.
+    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+    y64_3 = y64 + 2 * (y64 - y0.astype(np.float64))
+    err, worst = oracle.scaled_error(ref, y64_3, 3 * bound)
+    assert err <= 3 * TOL, f"scaled error {err:.3e} at row {worst}"
