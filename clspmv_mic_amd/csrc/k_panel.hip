@@ -9,14 +9,16 @@
 // in-panel column and a segment-start bit in 16 bits, so the matrix stream
 // is 6 bytes per nonzero instead of 12 (COO) or 8 (CSR).
 //
-// Pass 1: each wave takes sub-chunks of 256 entries (4 per lane), forms the
-// products and reduces every segment (one row's run inside the sub-chunk)
-// with a segmented scan over the wave's lanes (DPP row shifts and
-// broadcasts); each segment's sum goes to seg_val.  Pass 2: one work-group
+// Pass 1: each wave takes sub-chunks of 512 entries (8 consecutive ones per
+// lane), forms the products, sums each lane's entries in order and reduces
+// every segment (one row's run inside the sub-chunk) with a segmented scan
+// over the wave's lanes (DPP row shifts and broadcasts); each segment's sum
+// goes to seg_val.  Eight entries per lane spread the scan's fixed cost
+// over twice the entries of a 4-per-lane layout.  Pass 2: one work-group
 // per row block loads the block's segments (one contiguous range), puts
 // each at its slot of a row-major LDS layout, and each row adds its slots
-// in panel order to y (long rows spread over the work-group).  The summation order is fixed by the encoding; no
-// atomics.
+// in panel order to y (long rows spread over the work-group).  The
+// summation order is fixed by the encoding; no atomics.
 
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -26,9 +28,11 @@ namespace {
 
 using namespace dev;
 
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kU = 4;  // sub-chunks in flight per wave
+constexpr int kPer = launch::kPanelSub / kWave;  // entries per lane
+static_assert(kPer == 8, "pass 1 is written for 8 entries per lane");
+constexpr int kU = 2;  // sub-chunks in flight per wave
 
 // DPP controls (GFX9 encoding): shifts within 16-lane rows, broadcasts of
 // lane 15 / 31 to the following rows, and a whole-wave shift by one.
@@ -57,34 +61,81 @@ __device__ __forceinline__ void seg_step(float& s, int& g) {
     g |= tg;
 }
 
-struct SubChunk {
-    u32x2 c;   // four 16-bit entry words
-    f32x4 v;   // four values
+// A lane's eight values as loaded (converted to FP32 only when used, so a
+// load is never waited for before its sub-chunk's turn).
+template <class V>
+struct Vals8;
+template <>
+struct Vals8<ValF32> {
+    f32x4 a, b;
+    __device__ __forceinline__ void load(const float* __restrict__ p) {
+        a = *reinterpret_cast<const f32x4*>(p);
+        b = *reinterpret_cast<const f32x4*>(p + 4);
+    }
+    __device__ __forceinline__ float get(int k) const { return k < 4 ? a[k] : b[k - 4]; }
+};
+template <class Cvt>
+struct Vals8<Val16<Cvt>> {
+    u32x4 w;
+    __device__ __forceinline__ void load(const unsigned short* __restrict__ p) {
+        w = *reinterpret_cast<const u32x4*>(p);
+    }
+    __device__ __forceinline__ float get(int k) const {
+        const unsigned v = w[k >> 1];
+        return Cvt::f((unsigned short)(k & 1 ? v >> 16 : v & 0xFFFFu));
+    }
 };
 
-// Reduces one sub-chunk (4 entries per lane) to its segment sums.  xs holds
+template <class V>
+struct SubChunk {
+    u32x4 c;     // eight 16-bit entry words
+    Vals8<V> v;  // eight values
+};
+
+// One lane's share of a sub-chunk's segment sums, held in registers until
+// they are stored.  Its segments are consecutive: `before` - 1 (the one
+// open from earlier lanes, closed at the lane's first flag with value
+// `first`), then one per further flag (the one closed by flag k has the
+// running sum r[k - 1]), and, in the last lane, the sub-chunk's last
+// segment (value `last`).  Bit k of `flags` is entry k's segment-start bit.
+struct SegSums {
+    float first, r[kPer - 1], last;
+    int before, flags;
+};
+
+// Reduces one sub-chunk (8 entries per lane) to its segment sums.  xs holds
 // the panel's x and a 0 at index pad (the panel width): padding entries
 // (column word 0x7FFF >= pad) gather it, so every lane reads x with no
-// branch and the four LDS reads are in flight together.
-__device__ __forceinline__ void seg_scan_write(const SubChunk& q, const float* __restrict__ xs,
-                                               unsigned pad, int lane, float* __restrict__ out) {
-    const unsigned w[4] = {q.c.x & 0xFFFFu, q.c.x >> 16, q.c.y & 0xFFFFu, q.c.y >> 16};
-    const float v[4] = {q.v.x, q.v.y, q.v.z, q.v.w};
-    int f[4];
-    float xv[4];
+// branch and the LDS reads are in flight together.
+template <class V>
+__device__ __forceinline__ void seg_scan(const SubChunk<V>& q, const float* __restrict__ xs, unsigned pad,
+                                         int lane, SegSums& o) {
+    unsigned w[kPer];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int j = 0; j < kPer / 2; ++j) {
+        w[2 * j] = q.c[j] & 0xFFFFu;
+        w[2 * j + 1] = q.c[j] >> 16;
+    }
+    int f[kPer];
+    float xv[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
         f[k] = (int)(w[k] >> 15);
         xv[k] = xs[min(w[k] & 0x7FFFu, pad)];
     }
     if (lane == 0) f[0] = 1;  // every sub-chunk opens a segment
     // Running sums inside the lane, restarted at each flag (one rounding
     // per entry: the product is fused into the add).
-    float r[4];
-    r[0] = v[0] * xv[0];
+    float r[kPer];
+    r[0] = q.v.get(0) * xv[0];
 #pragma unroll
-    for (int k = 1; k < 4; ++k) r[k] = fmaf(v[k], xv[k], f[k] ? 0.0f : r[k - 1]);
-    const int c0 = f[0], c1 = c0 + f[1], c2 = c1 + f[2], nf = c2 + f[3];
+    for (int k = 1; k < kPer; ++k) r[k] = fmaf(q.v.get(k), xv[k], f[k] ? 0.0f : r[k - 1]);
+    int nf = 0, flags = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        nf += f[k];
+        flags |= f[k] << k;
+    }
     // Segment numbering: the flags of earlier lanes (an exclusive scan of
     // nf over the wave: DPP row shifts, then the row broadcasts), then this
     // lane's own.
@@ -95,17 +146,15 @@ __device__ __forceinline__ void seg_scan_write(const SubChunk& q, const float* _
     incl += dpp_int<kRowShr8, 0xF>(incl);
     incl += dpp_int<kRowBcast15, 0xA>(incl);
     incl += dpp_int<kRowBcast31, 0xC>(incl);
-    const int before = incl - nf;
-    // Segments that open and close inside the lane.
-    if (f[1] && c0) out[before + c0 - 1] = r[0];
-    if (f[2] && c1) out[before + c1 - 1] = r[1];
-    if (f[3] && c2) out[before + c2 - 1] = r[2];
     // The lane's head: its entries before its first flag.
-    const float head = f[0] ? 0.0f : f[1] ? r[0] : f[2] ? r[1] : f[3] ? r[2] : r[3];
+    float head = r[kPer - 1];
+#pragma unroll
+    for (int k = kPer - 1; k >= 1; --k) head = f[k] ? r[k - 1] : head;
+    if (f[0]) head = 0.0f;
     // Segmented inclusive scan over lanes of the amount each lane passes on:
-    // its tail from the last flag, or its whole sum when it has none -- r[3]
-    // either way.
-    float s = r[3];
+    // its tail from the last flag, or its whole sum when it has none -- the
+    // last running sum either way.
+    float s = r[kPer - 1];
     int g = nf ? 1 : 0;
     seg_step<kRowShr1, 0xF>(s, g);
     seg_step<kRowShr2, 0xF>(s, g);
@@ -114,16 +163,41 @@ __device__ __forceinline__ void seg_scan_write(const SubChunk& q, const float* _
     seg_step<kRowBcast15, 0xA>(s, g);
     seg_step<kRowBcast31, 0xC>(s, g);
     // The segment open before this lane ends at its first flag.
-    const float prev = dpp_float<kWaveShr1, 0xF>(s);
-    if (nf && lane > 0) out[before - 1] = prev + head;
+    o.first = dpp_float<kWaveShr1, 0xF>(s) + head;
+#pragma unroll
+    for (int k = 0; k < kPer - 1; ++k) o.r[k] = r[k];
+    o.last = s;
+    o.before = incl - nf;
+    o.flags = flags;
+}
+
+// out[i] for a small non-negative i: a 32-bit byte offset from the
+// wave-uniform base, so the store needs no 64-bit address arithmetic.
+__device__ __forceinline__ void store_at(float* __restrict__ out, int i, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(out) + (unsigned)(i << 2)) = v;
+}
+
+// Stores one lane's segment sums (predicated, in-lane).  Indices are below
+// kPanelSub + 1: a sub-chunk's segments start at out.
+__device__ __forceinline__ void seg_store(const SegSums& o, int lane, float* __restrict__ out) {
+    const int b = o.before;
+    // Segments that open and close inside the lane: flag k closes the one
+    // opened by the lane's previous flag, numbered by the flags before k.
+#pragma unroll
+    for (int k = 1; k < kPer; ++k) {
+        const int below = o.flags & ((1 << k) - 1);
+        if ((o.flags >> k) & 1 && below) store_at(out, b + __builtin_popcount(below) - 1, o.r[k - 1]);
+    }
+    const int nf = __builtin_popcount(o.flags);
+    if (nf && lane > 0) store_at(out, b - 1, o.first);
     // The last lane closes the sub-chunk's last segment.
-    if (lane == kWave - 1) out[before + nf - 1] = s;
+    if (lane == kWave - 1) store_at(out, b + nf - 1, o.last);
 }
 
 template <int NT, class V>
 __global__ __launch_bounds__(NT) void panel_pass1_kernel(
     const int4* __restrict__ wg, int pw, int cols,
-    const unsigned short* __restrict__ pcol, const typename V::T* __restrict__ pval,
+    const unsigned short* __restrict__ pcol, const typename V::T* __restrict__ pval, int pad_at,
     const int* __restrict__ sc_seg, const int* __restrict__ sc_start,
     const float* __restrict__ x, bool xa, float* __restrict__ seg_val) {
     extern __shared__ float xs[];
@@ -182,43 +256,71 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     const int lane = tid & (kWave - 1);
     constexpr int nw = NT / kWave;
-    // Software pipeline: the loads of the next kU sub-chunks are in flight
-    // while the current ones are reduced.
-    SubChunk cur[kU], nxt[kU];
-    auto load = [&](SubChunk* q, int sc) {
+    // Every lane loads on every path: lanes past a short sub-chunk's end,
+    // and whole sub-chunks past the work-group's last, read the padding
+    // block, which contributes nothing.  With no load under a branch the
+    // count of memory operations in flight is the same on every path, so
+    // each wait covers just the group it needs (a load that might have
+    // been skipped makes the compiler wait for everything).
+    auto load = [&](SubChunk<V>* q, int sc) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int c = sc + u * nw;
-            // Lanes past a short sub-chunk's end hold padding.
-            q[u].c = u32x2{0x7FFF7FFFu, 0x7FFF7FFFu};
-            q[u].v = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            int i = pad_at;
             if (c < sc1) {
-                const int i0 = s_start[c - sc0] + lane * 4;
-                if (i0 < s_start[c - sc0 + 1]) {
-                    // Plain (cached) loads: measured faster here than
-                    // non-temporal ones, which the x panel does not need.
-                    q[u].c = *reinterpret_cast<const u32x2*>(pcol + i0);
-                    const float4 v = V::ld4c(pval + i0);
-                    q[u].v = f32x4{v.x, v.y, v.z, v.w};
-                }
+                const int i0 = s_start[c - sc0] + lane * kPer;
+                if (i0 < s_start[c - sc0 + 1]) i = i0;
             }
+            // Plain (cached) loads: measured faster here than non-temporal
+            // ones, which the x panel does not need.
+            q[u].c = *reinterpret_cast<const u32x4*>(pcol + i);
+            q[u].v.load(pval + i);
         }
     };
-    int sc = sc0 + wave;
-    load(cur, sc);
-    for (; sc < sc1; sc += nw * kU) {
-        load(nxt, sc + nw * kU);
+    // The sums of one group of sub-chunks are stored when the next group
+    // starts, before its successor's loads are issued.  Memory operations
+    // retire in issue order and count together, so the wait for a group's
+    // loads then covers only stores issued earlier, which have had a whole
+    // iteration to complete; stores issued after the loads would add their
+    // latency to every wait.
+    SegSums res[kU];
+    auto store = [&](int base) {
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            const int c = sc + u * nw;
+            const int c = base + u * nw;
             // The segment base is wave-uniform: a scalar base lets every
             // store take a 32-bit lane offset.
-            if (c < sc1)
-                seg_scan_write(cur[u], xs, (unsigned)pw, lane,
-                               seg_val + __builtin_amdgcn_readfirstlane(s_seg[c - sc0]));
+            if (c < sc1) seg_store(res[u], lane, seg_val + __builtin_amdgcn_readfirstlane(s_seg[c - sc0]));
         }
+    };
+    // Two register groups take turns (no copies between them): one is
+    // reduced while the other's loads are in flight.  The last group
+    // prefetches nothing, so no wave ends waiting for loads of padding.
+    constexpr int step = nw * kU;
+    int sc = sc0 + wave;
+    if (sc >= sc1) return;
+    bool pending = false;  // res holds the previous group's sums
+    // Reduces the group at sc (in cur), prefetching the next one into nxt;
+    // true once the last group is done.
+    auto group = [&](SubChunk<V>* cur, SubChunk<V>* nxt) -> bool {
+        if (pending) store(sc - step);
+        if (sc + step >= sc1) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
+            for (int u = 0; u < kU; ++u)
+                if (sc + u * nw < sc1) seg_scan(cur[u], xs, (unsigned)pw, lane, res[u]);
+            store(sc);
+            return true;
+        }
+        load(nxt, sc + step);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) seg_scan(cur[u], xs, (unsigned)pw, lane, res[u]);
+        pending = true;
+        sc += step;
+        return false;
+    };
+    SubChunk<V> qa[kU], qb[kU];
+    load(qa, sc);
+    while (!group(qa, qb) && !group(qb, qa)) {
     }
 }
 
@@ -314,8 +416,8 @@ hipError_t panel_prepare() {
 }
 
 hipError_t panel_pass1(hipStream_t s, int nwg, const int* wg, int pw, int cols, const unsigned short* pcol,
-                       int dtype, const void* pval, const int* sc_seg, const int* sc_start, const float* x,
-                       bool x_aligned, float* seg_val) {
+                       int dtype, const void* pval, int pad_at, const int* sc_seg, const int* sc_start,
+                       const float* x, bool x_aligned, float* seg_val) {
     if (nwg <= 0) return hipSuccess;
     if (pw <= 0 || pw > kPanelMax) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * ((size_t)pw + 1);  // x panel + the padding entries' 0
@@ -324,16 +426,16 @@ hipError_t panel_pass1(hipStream_t s, int nwg, const int* wg, int pw, int cols, 
     switch (dtype) {
         case 0:
             hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValF32>), dim3(nwg), dim3(kPanelThreads), lds,
-                               s, wg4, pw, cols, pcol, static_cast<const float*>(pval), sc_seg, sc_start, x,
+                               s, wg4, pw, cols, pcol, static_cast<const float*>(pval), pad_at, sc_seg, sc_start, x,
                                x_aligned, seg_val);
             break;
         case 1:
             hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValBF16>), dim3(nwg), dim3(kPanelThreads),
-                               lds, s, wg4, pw, cols, pcol, p16, sc_seg, sc_start, x, x_aligned, seg_val);
+                               lds, s, wg4, pw, cols, pcol, p16, pad_at, sc_seg, sc_start, x, x_aligned, seg_val);
             break;
         case 2:
             hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValF16>), dim3(nwg), dim3(kPanelThreads),
-                               lds, s, wg4, pw, cols, pcol, p16, sc_seg, sc_start, x, x_aligned, seg_val);
+                               lds, s, wg4, pw, cols, pcol, p16, pad_at, sc_seg, sc_start, x, x_aligned, seg_val);
             break;
         default:
             return hipErrorInvalidValue;

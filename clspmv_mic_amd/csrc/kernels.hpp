@@ -34,8 +34,9 @@ hipError_t coo_segreduce(hipStream_t s, int nnz, const int* row, const int* col,
 // ---- x-panel encoding (COO and CSR) -------------------------------------------
 // Columns are cut into panels of at most kPanelMax floats (a panel's x fits
 // in LDS) and rows into blocks.  Entries are grouped by (panel, row block),
-// row-sorted inside a group; a group is padded to a multiple of 4 entries
-// and cut into sub-chunks of kPanelSub entries (the last one shorter;
+// row-sorted inside a group; a group is padded to a multiple of 8 entries
+// (one pass-1 lane's share) and cut into sub-chunks of kPanelSub entries
+// (the last one shorter;
 // sc_start holds each sub-chunk's first entry).  An entry is a 16-bit word
 // (bit 15: first entry of a segment; bits 0-14: column inside the panel,
 // kPanelPad for padding) and a float.  A segment is one row's run inside
@@ -48,7 +49,7 @@ hipError_t coo_segreduce(hipStream_t s, int nnz, const int* row, const int* col,
 // reads its block's segments as one contiguous range, places each in LDS at
 // its precomputed slot (seg_dst) of the block's row-major layout (row_slot:
 // first slot per row), and adds each row's slots, in panel order, to y.
-constexpr int kPanelSub = 256;        // entries per wave-sized sub-chunk
+constexpr int kPanelSub = 512;        // entries per wave-sized sub-chunk (8 per lane)
 constexpr int kPanelMax = 32760;      // largest panel width (floats)
 constexpr unsigned kPanelPad = 0x7FFF;  // column code of padding entries
 constexpr int kPanelThreads = 1024;   // pass-1 work-group size
@@ -70,9 +71,10 @@ static_assert(kLongSlots <= kLongRowF64 / kPanelSub, "every row longer than kLon
 // and adds the row sums into y.  Pass 1 touches nothing pass 2 of another
 // matrix or any other kernel writes, so it may run on a second stream while
 // earlier kernels update y, as long as pass 2 waits for it.
+// pad_at: entry index of 8 padding entries (column code kPanelPad, value 0).
 hipError_t panel_pass1(hipStream_t s, int nwg, const int* wg, int pw, int cols, const unsigned short* pcol,
-                       int dtype, const void* pval, const int* sc_seg, const int* sc_start, const float* x,
-                       bool x_aligned, float* seg_val);
+                       int dtype, const void* pval, int pad_at, const int* sc_seg, const int* sc_start,
+                       const float* x, bool x_aligned, float* seg_val);
 hipError_t panel_pass2(hipStream_t s, const float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
                        const int* row_slot, const unsigned short* seg_dst, const int* blk_long,
                        const int* longs, float* y);

@@ -22,8 +22,8 @@ int auto_panel_width(int cols, long long nnz) {
 
 int panel_group_size(const std::vector<int>& sc, int slots, int pw) {
     // Staging a panel costs about as much as this many sub-chunks (pw floats
-    // from L2 against 1.5 KB of encoded entries per sub-chunk from HBM).
-    const long long stage = std::max(8, pw / 1024);
+    // from L2 against 6 bytes of encoded entry per nonzero from HBM).
+    const long long stage = std::max(4, pw / (4 * launch::kPanelSub));
     long long best = -1;
     int bw = launch::kPanelScPerWg;
     for (int W = 8; W <= launch::kPanelScPerWg; W += 8) {
@@ -62,7 +62,7 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
 
     // Row blocks for pass 2: at most kBlockRowsMax rows and, bounding each
     // row's segments by its per-panel runs (a run of L > 1 entries becomes
-    // at most L/256 + 2 segments), at most `cap` segments (a longer row
+    // at most L/kSub + 2 segments), at most `cap` segments (a longer row
     // alone, up to kBlockSegsMax).  Two pass-2 work-groups run per CU; cap
     // splits the segments evenly over whole rounds of them.
     if (rows >= (1 << 30)) return fail(CLSPMV_ERR_UNSUPPORTED, "panel encoding needs rows < 2^30");
@@ -116,16 +116,17 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
     for (int b = 0; b < nb; ++b)
         std::fill(blk_of.begin() + blk_row[b], blk_of.begin() + blk_row[b + 1], b);
 
-    // Groups (panel, block) in pass-1 order, each padded to a multiple of 4
-    // entries (one lane's load) and cut into sub-chunks of kSub entries, the
-    // last one shorter.
+    // Groups (panel, block) in pass-1 order, each padded to a multiple of
+    // kPer entries (one lane's load) and cut into sub-chunks of kSub entries,
+    // the last one shorter.
     const size_t G = (size_t)P * nb;
     std::vector<long long> cnt(G, 0);
     for (long long i = 0; i < nnz; ++i) cnt[(size_t)(col[i] / pw) * nb + blk_of[row[i]]]++;
     std::vector<long long> gstart(G + 1, 0);
-    for (size_t g = 0; g < G; ++g) gstart[g + 1] = gstart[g] + (cnt[g] + 3) / 4 * 4;
+    constexpr int kPer = kSub / 64;  // entries per pass-1 lane
+    for (size_t g = 0; g < G; ++g) gstart[g + 1] = gstart[g] + (cnt[g] + kPer - 1) / kPer * kPer;
     const long long np = gstart[G];
-    if (np >= INT_MAX)
+    if (np + kPer > INT_MAX)
         return fail(CLSPMV_ERR_TOO_LARGE, "panel encoding of %lld entries exceeds 32-bit indices", np);
     std::vector<int> sc_start;   // first entry of each sub-chunk (+ end)
     std::vector<int> g_sc(G + 1);  // first sub-chunk of each group
@@ -154,7 +155,7 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
     }
 
     // Segments: one at every sub-chunk start and at every row change.
-    // Padding (under 4 entries, at a group's end) never starts a sub-chunk,
+    // Padding (under kPer entries, at a group's end) never starts a sub-chunk,
     // so every segment has a row.  Count per group first.
     std::vector<int> gseg(G, 0);
     for (size_t g = 0; g < G; ++g)
@@ -236,11 +237,11 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
     blk_long[nb] = (int)(longs.size() / 4);
 
     // What the kernels rely on: every sub-chunk opens a segment, its
-    // segments' slots exist, and its lanes read whole 4-entry groups; each
+    // segments' slots exist, and its lanes read whole kPer-entry groups; each
     // block's segments fit one pass-2 work-group and belong to its rows.
     for (long long c = 0; c < nsc; ++c) {
         const int k0 = sc_start[c], k1 = sc_start[c + 1];
-        if (!(pc[k0] & 0x8000u) || k1 - k0 > kSub || (k1 - k0) % 4)
+        if (!(pc[k0] & 0x8000u) || k1 - k0 > kSub || (k1 - k0) % kPer)
             return fail(CLSPMV_ERR_INVALID, "panel encoding: malformed sub-chunk %lld", c);
         int nf = 0;
         for (int k = k0; k < k1; ++k) nf += (pc[k] >> 15) & 1;
@@ -290,6 +291,11 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
     d->pnsc = g_sc[G];
     d->pnp = P;
     d->pnb = nb;
+    // One lane's worth of padding after the last group: pass 1 points the
+    // loads of lanes with no entry at it.
+    pc.insert(pc.end(), kPer, (unsigned short)launch::kPanelPad);
+    pv.insert(pv.end(), kPer, 0.0f);
+    d->ppad = (int)np;
     int st = upload_array(d, wg.data(), wg.size(), &d->pwg);
     if (!st) st = upload_array(d, pc.data(), pc.size(), &d->pcol);
     if (!st && dtype == CLSPMV_DT_F32) {

@@ -105,7 +105,7 @@ bool dmat_has_prepass(const clspmv_dmat* A) {
 int dmat_prepass(hipStream_t s, const clspmv_dmat* A, const float* x) {
     if (!dmat_has_prepass(A)) return CLSPMV_OK;
     const hipError_t e = launch::panel_pass1(s, A->pnwg, A->pwg, A->pw, A->info.cols, A->pcol, A->dtype,
-                                             A->dtype ? (const void*)A->plpval : (const void*)A->pval,
+                                             A->dtype ? (const void*)A->plpval : (const void*)A->pval, A->ppad,
                                              A->psc_seg, A->psc_start, x, aligned16(x), A->pseg);
     if (e != hipSuccess)
         return fail(CLSPMV_ERR_HIP, "panel pass 1 failed (format %d): %s", A->info.format, hipGetErrorString(e));

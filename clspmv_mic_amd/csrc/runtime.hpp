@@ -46,6 +46,7 @@ struct clspmv_dmat {
     // x-panel encoding of COO / CSR (pw == 0: absent).  plain == false when
     // the (row, col, val) / CSR arrays were not uploaded.
     int pw = 0, pnwg = 0, pnp = 0, pnb = 0, pnsc = 0;
+    int ppad = 0;  // entry index of the padding (one lane's share) after the encoding
     int* pwg = nullptr;  // int4 per pass-1 work-group
     unsigned short* pcol = nullptr;
     float* pval = nullptr;

@@ -88,7 +88,7 @@ typedef struct {
     int    plain;           /* 1 when the format's own arrays are held  */
     int    dtype;           /* enum clspmv_dtype of the stored values    */
     int    panel_groups;    /* panel encoding: pass-1 work-groups        */
-    int    panel_subchunks; /* panel encoding: 256-entry sub-chunks      */
+    int    panel_subchunks; /* panel encoding: 512-entry sub-chunks      */
     int    panel_blocks;    /* panel encoding: pass-2 row blocks         */
 } clspmv_dmat_info;
 
@@ -142,7 +142,7 @@ enum clspmv_dtype { CLSPMV_DT_F32 = 0, CLSPMV_DT_BF16 = 1, CLSPMV_DT_F16 = 2 };
 typedef struct {
     int panel_width;
     int dtype;
-    int panel_group;  /* sub-chunks (256 entries) per pass-1 work-group: 8..512,
+    int panel_group;  /* sub-chunks (512 entries) per pass-1 work-group: 8..512,
                          or 0 to size the work-groups so that they fill the
                          GPU's CUs in whole rounds */
 } clspmv_upload_opts;

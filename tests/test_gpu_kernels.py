@@ -412,7 +412,9 @@ def test_panel_row_too_long_falls_back(ctx, oracle):
     is refused, the automatic choice keeps the plain arrays; both give the
     oracle's result where they run."""
     rng = np.random.default_rng(23)
-    n_long = 4_500_000
+    # About n_long / 512 segments (one per sub-chunk) against 16384 per
+    # pass-2 block.
+    n_long = 9_000_000
     rows = np.concatenate([np.zeros(n_long, np.int64), rng.integers(1, 2000, size=20000)])
     cols = rng.integers(0, 4_000_000, size=rows.size)
     A = cs.Coo(rows, cols, rng.standard_normal(rows.size).astype(np.float32), 2000, 4_000_000).sorted()
