@@ -216,6 +216,51 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
     const int tid = threadIdx.x;
     if (tid <= sc1 - sc0) s_start[tid] = sc_start[sc0 + tid];
     if (tid < sc1 - sc0) s_seg[tid] = sc_seg[sc0 + tid];
+    // Sub-chunk indices are wave-uniform: keep them in scalar registers.
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int lane = tid & (kWave - 1);
+    constexpr int nw = NT / kWave;
+    // Every lane loads on every path: lanes past a short sub-chunk's end,
+    // and whole sub-chunks past the work-group's last, read the padding
+    // block, which contributes nothing.  With no load under a branch the
+    // count of memory operations in flight is the same on every path, so
+    // each wait covers just the group it needs (a load that might have
+    // been skipped makes the compiler wait for everything).
+    auto issue = [&](SubChunk<V>& q, int i0, int i1) {  // the lane's entries from i0 if below i1
+        const int i = i0 < i1 ? i0 : pad_at;
+        // Plain (cached) loads: measured faster here than non-temporal
+        // ones, which the x panel does not need.
+        q.c = *reinterpret_cast<const u32x4*>(pcol + i);
+        q.v.load(pval + i);
+    };
+    auto load = [&](SubChunk<V>* q, int sc) {  // sub-chunk bounds from the LDS table
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int c = sc + u * nw;
+            int i0 = pad_at, i1 = 0;
+            if (c < sc1) {
+                i0 = s_start[c - sc0] + lane * kPer;
+                i1 = s_start[c - sc0 + 1];
+            }
+            issue(q[u], i0, i1);
+        }
+    };
+    // The first group's loads go out while the panel is staged, its bounds
+    // read from the global table (the LDS copy is not ready before the
+    // barrier).
+    SubChunk<V> qa[kU], qb[kU];
+    auto load_first = [&]() {
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int c = sc0 + wave + u * nw;
+            int i0 = pad_at, i1 = 0;
+            if (c < sc1) {
+                i0 = sc_start[c] + lane * kPer;
+                i1 = sc_start[c + 1];
+            }
+            issue(qa[u], i0, i1);
+        }
+    };
     // Stage the panel: every load is issued before the first LDS store, so
     // the work-group waits for one memory latency, not one per step.
     constexpr int kStage = (launch::kPanelMax + 4 * NT - 1) / (4 * NT);
@@ -229,6 +274,7 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
             const int i = min((tid + k * NT) * 4, w4 - 4);
             v[k] = *reinterpret_cast<const f32x4*>(x + xb + i);
         }
+        load_first();
 #pragma unroll
         for (int k = 0; k < kStage; ++k) {
             const int i = (tid + k * NT) * 4;
@@ -243,6 +289,7 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
             const int i = min(tid + k * NT, w - 1);
             v[k] = x[xb + i];
         }
+        load_first();
 #pragma unroll
         for (int k = 0; k < 4 * kStage; ++k) {
             const int i = tid + k * NT;
@@ -252,31 +299,6 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
     }
     __syncthreads();
 
-    // Sub-chunk indices are wave-uniform: keep them in scalar registers.
-    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
-    const int lane = tid & (kWave - 1);
-    constexpr int nw = NT / kWave;
-    // Every lane loads on every path: lanes past a short sub-chunk's end,
-    // and whole sub-chunks past the work-group's last, read the padding
-    // block, which contributes nothing.  With no load under a branch the
-    // count of memory operations in flight is the same on every path, so
-    // each wait covers just the group it needs (a load that might have
-    // been skipped makes the compiler wait for everything).
-    auto load = [&](SubChunk<V>* q, int sc) {
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int c = sc + u * nw;
-            int i = pad_at;
-            if (c < sc1) {
-                const int i0 = s_start[c - sc0] + lane * kPer;
-                if (i0 < s_start[c - sc0 + 1]) i = i0;
-            }
-            // Plain (cached) loads: measured faster here than non-temporal
-            // ones, which the x panel does not need.
-            q[u].c = *reinterpret_cast<const u32x4*>(pcol + i);
-            q[u].v.load(pval + i);
-        }
-    };
     // The sums of one group of sub-chunks are stored when the next group
     // starts, before its successor's loads are issued.  Memory operations
     // retire in issue order and count together, so the wait for a group's
@@ -318,8 +340,6 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
         sc += step;
         return false;
     };
-    SubChunk<V> qa[kU], qb[kU];
-    load(qa, sc);
     while (!group(qa, qb) && !group(qb, qa)) {
     }
 }
