@@ -146,7 +146,17 @@ def run_spmm(args):
         Xs = cs.convert_to_f32(Xb, dtype)
         dX = cs.DeviceBuffer(ctx, Xb)
         dY = cs.DeviceVector(ctx, k * ldy, np.zeros(k * ldy, np.float32))
-        M.spmm(k, dX, ldx, dY, ldy)
+        tiled = args.x_layout == "tiled"
+        if tiled:  # X handed over in 4-row tiles: packed once, outside the timed region
+            dX4 = cs.spmm_pack_x(ctx, k, A.width, dX, ldx)
+
+        def product():
+            if tiled:
+                M.spmm_tiled(k, dX4, dY, ldy)
+            else:
+                M.spmm(k, dX, ldx, dY, ldy)
+
+        product()
         ctx.sync()
         # Error per right-hand side against the FP64 SpMV on the GPU: of the
         # stored (rounded) values and x -- the kernel's own error -- and of
@@ -157,13 +167,23 @@ def run_spmm(args):
             errs.append(cs.verify_spmv(ctx, ref_st, cs.DeviceVector(ctx, ldx, Xs[j]), yj)["max_scaled_error"])
             qerrs.append(cs.verify_spmv(ctx, ref_in, cs.DeviceVector(ctx, ldx, X[j]), yj)["max_scaled_error"])
         for _ in range(args.warmup):
-            M.spmm(k, dX, ldx, dY, ldy)
+            product()
         ctx.sync()
         t1 = time.perf_counter()
         for _ in range(args.steps):
-            M.spmm(k, dX, ldx, dY, ldy)
+            product()
         ctx.sync()
         ms = (time.perf_counter() - t1) * 1e3 / args.steps
+        # The other X layout, timed the same way, for comparison.
+        other_ms = None
+        if tiled:
+            M.spmm(k, dX, ldx, dY, ldy)
+            ctx.sync()
+            t2 = time.perf_counter()
+            for _ in range(args.steps):
+                M.spmm(k, dX, ldx, dY, ldy)
+            ctx.sync()
+            other_ms = (time.perf_counter() - t2) * 1e3 / args.steps
         flops = 2.0 * A.nnz * k
         nbytes = spmm_bytes(blocks, brows, A.height, A.width, k)
         out = {
@@ -173,11 +193,13 @@ def run_spmm(args):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
             "data": "synthetic",
             "config": {"workload": "4x4-block matrix, 524,288 block rows x 12 blocks, X uniform[-1,1]",
-                       "rows": A.height, "nnz": A.nnz, "blocks": blocks, "k": k},
+                       "rows": A.height, "nnz": A.nnz, "blocks": blocks, "k": k,
+                       "x_layout": "4-row tiles" if tiled else "column-major"},
             "roofline": {"bound": "hbm", "achieved": round(nbytes / (ms * 1e-3) / 1e9, 2), "peak": PEAK_HBM_GBPS,
                          "unit": "GB/s", "frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
                          "traffic": None, "algorithmic_bytes": nbytes},
             "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
+            "column_major_x_ms": round(other_ms, 6) if other_ms is not None else None,
             "error_vs_fp64_stored_values": [float(f"{e:.3e}") for e in errs],
             "error_vs_fp64_fp32_inputs": [float(f"{e:.3e}") for e in qerrs],
             "matrix_error_vs_fp64": {kk: (float(f"{v:.3e}") if isinstance(v, float) else v)
@@ -364,6 +386,8 @@ def main():
     ap.add_argument("--dtype", default="bf16", choices=["bf16", "fp16"], help="spmm: value type")
     ap.add_argument("--k", type=int, nargs="+", default=[8, 16, 32], help="spmm: right-hand sides")
     ap.add_argument("--err-columns", type=int, default=4, help="spmm: columns checked against FP64")
+    ap.add_argument("--x-layout", default="tiled", choices=["tiled", "column"],
+                    help="spmm: X in 4-row tiles (clspmv_spmm_tiled) or column-major (clspmv_spmm)")
     ap.add_argument("--matrices", type=int, default=64, help="serve: resident matrices")
     ap.add_argument("--requests", type=int, default=20000, help="serve: requests")
     ap.add_argument("--baseline-requests", type=int, default=2000, help="serve: plain-launch sample")

@@ -257,6 +257,8 @@ _sig("clspmv_spmv_multi", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int)
 _sig("clspmv_verify_spmv", C.c_int, VP, VP, VP, VP, VP, C.c_double, P(ErrorReport))
 _sig("clspmv_dmat_error_report", C.c_int, VP, VP, VP, C.c_ulonglong, C.c_double, P(ErrorReport))
 _sig("clspmv_spmm", C.c_int, VP, VP, C.c_int, VP, C.c_int, VP, C.c_int)
+_sig("clspmv_spmm_tiled", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int)
+_sig("clspmv_spmm_pack_x", C.c_int, VP, C.c_int, C.c_int, VP, C.c_int, VP)
 _sig("clspmv_convert_from_f32", C.c_int, PF, VP, C.c_size_t, C.c_int)
 _sig("clspmv_convert_to_f32", C.c_int, VP, PF, C.c_size_t, C.c_int)
 _sig("clspmv_server_create", C.c_int, VP, P(VP))
@@ -652,6 +654,14 @@ class DeviceBuffer:
             pass
 
 
+def spmm_pack_x(ctx: Context, k: int, cols: int, X: DeviceBuffer, ldx: int) -> DeviceBuffer:
+    """Column-major 16-bit X (k columns, leading dimension ldx) into the
+    4-row tiles of clspmv_spmm_tiled, on the GPU (clspmv_spmm_pack_x)."""
+    X4 = DeviceBuffer(ctx, np.zeros(max(1, (cols + 3) // 4 * k * 4), np.uint16))
+    check(lib.clspmv_spmm_pack_x(ctx.h, k, cols, X.p, ldx, X4.p), "spmm_pack_x")
+    return X4
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("device_ordinal", C.c_int), ("streams", C.c_int), ("max_batch", C.c_int),
                 ("hbm_budget", C.c_size_t), ("use_graphs", C.c_int)]
@@ -734,6 +744,11 @@ class BlockSpmm:
 
     def spmm(self, k: int, X, ldx: int, Y, ldy: int):
         check(lib.clspmv_spmm(self.ctx.h, self.h, k, X.p, ldx, Y.p, ldy), "spmm")
+
+    def spmm_tiled(self, k: int, X4, Y, ldy: int):
+        """Y += A X with X in 4-row tiles (clspmv_spmm_tiled; spmm_pack_x
+        builds them from a column-major X)."""
+        check(lib.clspmv_spmm_tiled(self.ctx.h, self.h, k, X4.p, Y.p, ldy), "spmm_tiled")
 
     def error_report(self, ref: "DeviceMatrix", seed: int = 1, tol: float = 1e-5) -> dict:
         """Error of one SpMM column (x rounded to the matrix's type) against

@@ -11,6 +11,11 @@
 //   B[k][j]: lane 4g+j, element k      -> X rows c0..c0+3 of column j
 //   D[i][j]: lane 4g+j, register i     -> Y rows r0..r0+3 of column j
 // so with X and Y column-major every lane moves whole 8- or 16-byte pieces.
+// X may instead come in 4-row tiles (TILED): the four 16-bit values of rows
+// 4c'..4c'+3 of right-hand side j at X[(c' * kt + j) * 4], so the four lanes
+// of a group read one 32-byte run per quad instead of four columns that lie
+// megabytes apart -- a quarter of the cache lines per block for the texture
+// path, which bounds the column-major form.
 // A group owns one block row and all right-hand sides; each Y element has
 // one owner, so Y is updated without atomics and the summation order
 // (blocks in storage order, four products per instruction) is fixed.
@@ -59,7 +64,9 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
 // FP32 MFMA accumulators are added into FP64 ones every kSpmmChunk blocks,
 // so a row's rounding error grows with the chunk, not the row length, and
 // y receives one rounding.
-template <bool BF16, int NQ, bool LONG>
+// ldx: X's leading dimension, or for TILED the tile stride in right-hand
+// sides (the k of the whole tiled X).
+template <bool BF16, int NQ, bool LONG, bool TILED>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int brows, int rows, int cols, int k, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
@@ -94,7 +101,13 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
         for (int q = 0; q < NQ; ++q) {
             const int xc = q * 4 + t;
             b[q] = s16x4{0, 0, 0, 0};
-            if (has && xc < k) b[q] = ld4(X + (size_t)xc * ldx + c0, cols - c0);
+            if constexpr (TILED) {
+                // Whole tiles: the tiled X is zero past the last column.
+                if (has && xc < k)
+                    b[q] = *reinterpret_cast<const s16x4*>(X + ((size_t)(c0 >> 2) * ldx + xc) * 4);
+            } else {
+                if (has && xc < k) b[q] = ld4(X + (size_t)xc * ldx + c0, cols - c0);
+            }
         }
     };
     // Software pipeline: block it + 1's loads are in flight while block it
@@ -159,39 +172,58 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     }
 }
 
-template <bool BF16, int NQ>
+template <bool BF16, int NQ, bool TILED>
 hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                      const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
                      float* Y, int ldy) {
     const long long slices = (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice;
     const long long blocks = (slices * kWave + kBlock - 1) / kBlock;
     if (long_rows)
-        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, true>), dim3((unsigned)blocks), dim3(kBlock), 0, s, brows,
-                           rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, true, TILED>), dim3((unsigned)blocks), dim3(kBlock), 0, s,
+                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
     else
-        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, brows,
-                           rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, false, TILED>), dim3((unsigned)blocks), dim3(kBlock), 0, s,
+                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
     return hipGetLastError();
 }
 
-template <bool BF16>
+template <bool BF16, bool TILED>
 hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                     const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
                     float* Y, int ldy) {
     // Up to 32 right-hand sides per launch (8 quads of accumulators).
     for (int c = 0; c < k; c += 32) {
         const int kc = std::min(32, k - c);
-        const unsigned short* Xc = X + (size_t)c * ldx;
+        // Right-hand side c: column c, or offset c inside every tile.
+        const unsigned short* Xc = X + (TILED ? (size_t)c * 4 : (size_t)c * ldx);
         float* Yc = Y + (size_t)c * ldy;
         const int nq = (kc + 3) / 4;
         hipError_t e;
-        if (nq <= 1) e = launch_nq<BF16, 1>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
-        else if (nq <= 2) e = launch_nq<BF16, 2>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
-        else if (nq <= 4) e = launch_nq<BF16, 4>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
-        else e = launch_nq<BF16, 8>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        if (nq <= 1)
+            e = launch_nq<BF16, 1, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else if (nq <= 2)
+            e = launch_nq<BF16, 2, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else if (nq <= 4)
+            e = launch_nq<BF16, 4, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else
+            e = launch_nq<BF16, 8, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// X (column-major, leading dimension ldx) into 4-row tiles; one thread per
+// (tile, right-hand side), zeros past the last column.
+__global__ void pack_x4_kernel(int k, int cols, int tiles, const unsigned short* __restrict__ X, int ldx,
+                               unsigned short* __restrict__ X4) {
+    const long long n = (long long)tiles * k;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(t % k), c = (int)(t / k) * 4;
+        const unsigned short* src = X + (size_t)j * ldx + c;
+        s16x4 v = ld4(src, cols - c);
+        *reinterpret_cast<s16x4*>(X4 + (size_t)t * 4) = v;
+    }
 }
 
 }  // namespace
@@ -200,11 +232,23 @@ namespace launch {
 
 hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows, int cols, int k,
                   const int* rp, const int* bcol, const unsigned short* val,
-                  const unsigned short* X, int ldx, float* Y, int ldy) {
+                  const unsigned short* X, int ldx, float* Y, int ldy, bool tiled) {
     if (brows <= 0 || k <= 0) return hipSuccess;
     if ((long long)brows * 4 / kBlock + 1 > 0x7FFFFFFFll) return hipErrorInvalidValue;
-    return bf16 ? launch_k<true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy)
-                : launch_k<false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    if (tiled)
+        return bf16 ? launch_k<true, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy)
+                    : launch_k<false, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    return bf16 ? launch_k<true, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy)
+                : launch_k<false, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+}
+
+hipError_t pack_x4(hipStream_t s, int k, int cols, const unsigned short* X, int ldx, unsigned short* X4) {
+    if (k <= 0 || cols <= 0) return hipSuccess;
+    const int tiles = (cols + 3) / 4;
+    const long long n = (long long)tiles * k;
+    const int grid = (int)std::min<long long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(pack_x4_kernel, dim3(grid), dim3(256), 0, s, k, cols, tiles, X, ldx, X4);
+    return hipGetLastError();
 }
 
 }  // namespace launch

@@ -126,15 +126,21 @@ hipError_t csr_multi(hipStream_t s, int nblocks, const int* blocks, const int* r
 // slice s's first slot (its width is (sptr[s+1] - sptr[s]) / kSpmmSlice),
 // slot sptr[s] + j * kSpmmSlice + g holds block j of row s * kSpmmSlice + g:
 // its element column in scol (a multiple of 4, -1 for padding) and its 16
-// BF16 / FP16 values, row-major, in val.  X: 16-bit, column-major (ldx);
-// Y: FP32 column-major (ldy).  ldx, ldy multiples of 4 and X, Y 16-byte
-// aligned.  long_rows: some block row holds more than kLongRowF64 / 4
-// blocks; the MFMA sums are then carried in FP64 every kSpmmChunk blocks.
+// BF16 / FP16 values, row-major, in val.  X: 16-bit, column-major (ldx),
+// or with `tiled` in 4-row tiles (value r of rows 4c..4c+3 of right-hand
+// side j at X[(c * ldx + j) * 4 + r], ldx then the tiles' right-hand-side
+// count, zeros past the last column); Y: FP32 column-major (ldy).  ldx, ldy
+// multiples of 4 and X, Y 16-byte aligned.  long_rows: some block row holds
+// more than kLongRowF64 / 4 blocks; the MFMA sums are then carried in FP64
+// every kSpmmChunk blocks.
 constexpr int kSpmmChunk = 16;
 constexpr int kSpmmSlice = 16;  // block rows per slice: one wave, 4 lanes per block row
 hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows, int cols, int k,
                   const int* sptr, const int* scol, const unsigned short* val,
-                  const unsigned short* X, int ldx, float* Y, int ldy);
+                  const unsigned short* X, int ldx, float* Y, int ldy, bool tiled = false);
+// Column-major X (k columns of `cols` values, leading dimension ldx) into
+// the tiled layout above (ceil(cols / 4) * k tiles of 4 values).
+hipError_t pack_x4(hipStream_t s, int k, int cols, const unsigned short* X, int ldx, unsigned short* X4);
 
 // ---- bandwidth probe ---------------------------------------------------------
 // dst[0..4*n4) = src[0..4*n4) with 128-bit loads/stores (16-byte aligned).

@@ -861,6 +861,38 @@ int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, i
     return CLSPMV_OK;
 }
 
+int clspmv_spmm_tiled(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X4, float* d_Y, int ldy) {
+    if (!ctx || !A || (!d_X4 && k) || (!d_Y && k)) return fail(CLSPMV_ERR_INVALID, "null argument");
+    const clspmv_dmat_info& I = A->info;
+    if (I.format != CLSPMV_FMT_BSPMM)
+        return fail(CLSPMV_ERR_UNSUPPORTED, "clspmv_spmm_tiled needs a matrix from clspmv_upload_bcsr_lowp");
+    if (k < 0) return fail(CLSPMV_ERR_INVALID, "k = %d", k);
+    if (ldy < I.rows || ldy % 4) return fail(CLSPMV_ERR_INVALID, "ldy %d: need >= rows and a multiple of 4", ldy);
+    if (((uintptr_t)d_X4 & 7) || ((uintptr_t)d_Y & 15))
+        return fail(CLSPMV_ERR_INVALID, "X must be 8-byte and Y 16-byte aligned");
+    int st = use_ctx_device(ctx);
+    if (st) return st;
+    if (k == 0 || I.rows == 0) return CLSPMV_OK;
+    hipError_t e = launch::bspmm4(ctx->stream, A->dtype == CLSPMV_DT_BF16, A->long_brows, A->brows, I.rows, I.cols, k,
+                                  A->rp, A->col, A->lpval, static_cast<const unsigned short*>(d_X4), k, d_Y, ldy,
+                                  true);
+    if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "spmm launch failed: %s", hipGetErrorString(e));
+    return CLSPMV_OK;
+}
+
+int clspmv_spmm_pack_x(clspmv_ctx* ctx, int k, int cols, const void* d_X, int ldx, void* d_X4) {
+    if (!ctx || (k > 0 && cols > 0 && (!d_X || !d_X4))) return fail(CLSPMV_ERR_INVALID, "null argument");
+    if (k < 0 || cols < 0) return fail(CLSPMV_ERR_INVALID, "k = %d, cols = %d", k, cols);
+    if (ldx < cols || ldx % 4) return fail(CLSPMV_ERR_INVALID, "ldx %d: need >= cols and a multiple of 4", ldx);
+    if (((uintptr_t)d_X & 7) || ((uintptr_t)d_X4 & 7)) return fail(CLSPMV_ERR_INVALID, "X and X4 must be 8-byte aligned");
+    int st = use_ctx_device(ctx);
+    if (st) return st;
+    const hipError_t e = launch::pack_x4(ctx->stream, k, cols, static_cast<const unsigned short*>(d_X), ldx,
+                                         static_cast<unsigned short*>(d_X4));
+    if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "pack launch failed: %s", hipGetErrorString(e));
+    return CLSPMV_OK;
+}
+
 int clspmv_spmv_multi(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const float* d_X, float* d_Y,
                       int ldy) {
     if (!ctx || !A || (k > 0 && (!d_X || !d_Y))) return fail(CLSPMV_ERR_INVALID, "null argument");

@@ -193,6 +193,15 @@ int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
  *   4; X 8-byte and Y 16-byte aligned.  Asynchronous on the context's
  *   stream.  Each Y element is written by one lane: no atomics, a fixed
  *   summation order.
+ * clspmv_spmm_tiled: the same product with X in 4-row tiles: the values of
+ *   rows 4c..4c+3 of right-hand side j at X4[(c * k + j) * 4 + 0..3], for
+ *   c < ceil(cols / 4), zeros past the last column (8-byte aligned).  A
+ *   lane group then reads one contiguous run per 4 right-hand sides instead
+ *   of 4 columns, a quarter of the cache lines of the column-major form;
+ *   the results are identical to clspmv_spmm's (same products, same order).
+ * clspmv_spmm_pack_x: column-major X (k columns of `cols` values, leading
+ *   dimension ldx, a multiple of 4) into that tiled layout, on the GPU
+ *   (asynchronous on the context's stream).
  * clspmv_convert_from_f32 / clspmv_convert_to_f32: host conversions of n
  *   values between float and dtype (round to nearest even), for building X
  *   and for exact references. */
@@ -200,6 +209,8 @@ int clspmv_upload_bcsr_lowp(clspmv_ctx* ctx, const clspmv_bcsr_matrix* m, int dt
                             clspmv_dmat** out);
 int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, int ldx,
                 float* d_Y, int ldy);
+int clspmv_spmm_tiled(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X4, float* d_Y, int ldy);
+int clspmv_spmm_pack_x(clspmv_ctx* ctx, int k, int cols, const void* d_X, int ldx, void* d_X4);
 int clspmv_convert_from_f32(const float* src, void* dst, size_t n, int dtype);
 int clspmv_convert_to_f32(const void* src, float* dst, size_t n, int dtype);
 

@@ -91,3 +91,53 @@ def test_spmm_argument_checks(ctx):
         y = cs.DeviceVector(ctx, A.height)
         check = cs.check
         check(cs.lib.clspmv_spmv(ctx.h, M.h, 0, x.p, y.p), "spmv")
+
+
+def _tiles(Xb: np.ndarray, cols: int) -> np.ndarray:
+    """Host restatement of the 4-row tile layout: value r of rows 4c..4c+3
+    of right-hand side j at [(c * k + j) * 4 + r], zeros past `cols`."""
+    k = Xb.shape[0]
+    nt = (cols + 3) // 4
+    full = np.zeros((k, nt * 4), np.uint16)
+    full[:, :cols] = Xb[:, :cols]
+    return np.ascontiguousarray(full.reshape(k, nt, 4).transpose(1, 0, 2)).reshape(-1)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("dtype", [cs.DT_BF16, cs.DT_F16])
+@pytest.mark.parametrize("k", [1, 3, 8, 12, 32, 40])
+def test_spmm_tiled_matches_column_major(ctx, name, dtype, k):
+    """The tiled-X entry point runs the same products in the same order as
+    clspmv_spmm: identical Y, bit for bit; the GPU packer builds the tiles
+    the host restatement does (odd column counts pad with zeros)."""
+    A = CASES[name]
+    rng = np.random.default_rng(100 + k)
+    H, W = A.height, A.width
+    ldx, ldy = (W + 3) // 4 * 4 + 4, (H + 3) // 4 * 4
+    X = rng.uniform(-1, 1, size=(k, ldx)).astype(np.float32)
+    Y0 = rng.uniform(-1, 1, size=(k, ldy)).astype(np.float32)
+    Xb = cs.convert_from_f32(X, dtype)
+    M = cs.BlockSpmm(ctx, cs.coo2bcsr(A.copy(), 4, 4), dtype)
+    dX = cs.DeviceBuffer(ctx, Xb)
+    dX4 = cs.spmm_pack_x(ctx, k, W, dX, ldx)
+    ctx.sync()
+    np.testing.assert_array_equal(dX4.download(), _tiles(Xb, W))
+    dY = cs.DeviceBuffer(ctx, Y0)
+    dY4 = cs.DeviceBuffer(ctx, Y0)
+    M.spmm(k, dX, ldx, dY, ldy)
+    M.spmm_tiled(k, dX4, dY4, ldy)
+    ctx.sync()
+    np.testing.assert_array_equal(dY4.download(), dY.download())
+
+
+def test_spmm_tiled_argument_checks(ctx):
+    A = CASES["rand"]
+    M = cs.BlockSpmm(ctx, cs.coo2bcsr(A.copy(), 4, 4), cs.DT_BF16)
+    dX4 = cs.DeviceBuffer(ctx, np.zeros((A.width + 3) // 4 * 4 * 4, np.uint16))
+    dY = cs.DeviceBuffer(ctx, np.zeros(4 * ((A.height + 3) // 4 * 4), np.float32))
+    with pytest.raises(cs.ClspmvError):
+        M.spmm_tiled(4, dX4, dY, A.height + 1)       # ldy not a multiple of 4
+    dm = cs.DeviceMatrix(ctx, cs.coo2csr(A.copy()))
+    with pytest.raises(cs.ClspmvError):
+        lib_spmm = cs.lib.clspmv_spmm_tiled(ctx.h, dm.h, 4, dX4.p, dY.p, (A.height + 3) // 4 * 4)
+        cs.check(lib_spmm, "spmm_tiled")             # not a BSPMM matrix
