@@ -31,6 +31,7 @@ namespace {
 using namespace dev;
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
 
 template <bool BF16>
@@ -65,30 +66,39 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
 // so a row's rounding error grows with the chunk, not the row length, and
 // y receives one rounding.
 // ldx: X's leading dimension, or for TILED the tile stride in right-hand
-// sides (the k of the whole tiled X).
-template <bool BF16, int NQ, bool LONG, bool TILED>
+// sides (the k of the whole tiled X).  With TILED, lane t of a group takes
+// right-hand sides t * NQ + q (q < NQ), adjacent tiles, so one 16-byte load
+// serves two of them when the stride is even (X16); a right-hand side past
+// k reads some in-bounds tile of the chunk -- its column of the product is
+// never stored, and an MFMA output column depends only on its own B column.
+template <bool BF16, int NQ, bool LONG, bool TILED, bool X16>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int brows, int rows, int cols, int k, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
     const unsigned short* __restrict__ X, int ldx, float* __restrict__ Y, int ldy) {
+    static_assert(!X16 || (TILED && NQ >= 2), "16-byte loads take two tiles of the tiled layout");
     static_assert(launch::kSpmmSlice * 4 == kWave, "a wave holds one slice, four lanes per block row");
     const int lane = threadIdx.x & (kWave - 1);
     const int t = lane & 3, g = lane >> 2;
     // One wave per slice, in XCD-aware order: neighbouring block rows, which
     // share X rows, run on one XCD and share its L2.
-    const int sl = (int)(((long long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) / kWave);
+    // (Wave-uniform: in scalar registers, so the loop's exits are scalar
+    // branches.)
+    const int sl = __builtin_amdgcn_readfirstlane(
+        (int)(((long long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) / kWave));
     if (sl >= (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice) return;  // the whole wave
     const int br = sl * launch::kSpmmSlice + g;
     const bool active = br < brows;
     // The instruction is wave-wide: every row of the slice steps through the
     // slice's width, padding slots feeding zeros.
-    const int base = rp[sl];
-    const int nmax = (rp[sl + 1] - base) / launch::kSpmmSlice;
+    const int base = __builtin_amdgcn_readfirstlane(rp[sl]);
+    const int nmax = __builtin_amdgcn_readfirstlane((rp[sl + 1] - base) / launch::kSpmmSlice);
     f32x4 acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // Operands of step `it` (zeros for padding slots and past the width).
-    auto fetch = [&](int it, s16x4& a, s16x4* b) {
+    // Column-major X: operands of step `it` (zeros for padding slots and
+    // past the width).
+    auto fetch = [&](int it, s16x4& a, s16x4* b, bool& use) {
         a = s16x4{0, 0, 0, 0};
         int c0 = -1;
         if (it < nmax) {
@@ -96,23 +106,18 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
             a = *reinterpret_cast<const s16x4*>(val + j * 16 + t * 4);  // padding holds zeros
             c0 = bcol[j];
         }
-        const bool has = c0 >= 0;
+        use = c0 >= 0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int xc = q * 4 + t;
             b[q] = s16x4{0, 0, 0, 0};
-            if constexpr (TILED) {
-                // Whole tiles: the tiled X is zero past the last column.
-                if (has && xc < k)
-                    b[q] = *reinterpret_cast<const s16x4*>(X + ((size_t)(c0 >> 2) * ldx + xc) * 4);
-            } else {
-                if (has && xc < k) b[q] = ld4(X + (size_t)xc * ldx + c0, cols - c0);
-            }
+            if (use && xc < k) b[q] = ld4(X + (size_t)xc * ldx + c0, cols - c0);
         }
     };
     // Software pipeline: block it + 1's loads are in flight while block it
     // is multiplied.
     s16x4 a_cur, a_nxt, b_cur[NQ], b_nxt[NQ];
+    bool use_cur, use_nxt;
     double acc64[NQ][4];  // used by LONG only
     if constexpr (LONG) {
 #pragma unroll
@@ -128,16 +133,79 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
             acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         }
     };
-    fetch(0, a_cur, b_cur);
-    for (int it = 0; it < nmax; ++it) {
-        fetch(it + 1, a_nxt, b_nxt);
+    if constexpr (TILED) {
+        // Two register sets, P for even steps and Q for odd ones, take
+        // turns with nothing copied between them: a step's block column is
+        // loaded two steps ahead, its A values and X tiles one step ahead.
+        // Every load is unconditional (steps past the end re-read the last
+        // slot and are never multiplied), so each wait names an exact count
+        // and the next step's loads stay in flight during the MFMAs.
+        if (nmax > 0) {
+            auto slot = [&](int s) {
+                return (size_t)base + (size_t)min(s, nmax - 1) * launch::kSpmmSlice + g;
+            };
+            auto ld_col = [&](int s) { return bcol[slot(s)]; };
+            auto ld_a = [&](int s) { return *reinterpret_cast<const s16x4*>(val + slot(s) * 16 + t * 4); };
+            auto ld_x = [&](int c0, s16x4* b) {
+                // Padding slots (c0 < 0) read tile 0 and are zeroed in mma.
+                const unsigned short* xt = X + (size_t)(c0 >= 0 ? c0 >> 2 : 0) * ldx * 4;
+                if constexpr (X16) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] = mfma4<BF16>(a_cur, b_cur[q], acc[q]);
-        a_cur = a_nxt;
+                    for (int p = 0; p < NQ / 2; ++p) {
+                        const int r = min(t * NQ + 2 * p, k - 2);
+                        const s16x8 v = *reinterpret_cast<const s16x8*>(xt + (size_t)r * 4);
+                        b[2 * p] = v.lo;
+                        b[2 * p + 1] = v.hi;
+                    }
+                } else {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) b_cur[q] = b_nxt[q];
-        if constexpr (LONG) {
-            if (it % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
+                    for (int q = 0; q < NQ; ++q)
+                        b[q] = *reinterpret_cast<const s16x4*>(xt + (size_t)min(t * NQ + q, k - 1) * 4);
+                }
+            };
+            auto mma = [&](int s, const s16x4& a, const s16x4* b, bool use) {
+                const s16x4 z = s16x4{0, 0, 0, 0}, au = use ? a : z;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) acc[q] = mfma4<BF16>(au, use ? b[q] : z, acc[q]);
+                if constexpr (LONG) {
+                    if (s % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
+                }
+            };
+            // Steps go in pairs with no exit inside a pair (a load used only
+            // past an exit would be moved down to its use): an odd last step
+            // pairs with one past the end, multiplied as zeros (which can
+            // turn a zero sum of -0 into +0; nothing else changes).
+            int cP = ld_col(0), cQ = ld_col(1);
+            s16x4 aP = ld_a(0), aQ = ld_a(1);
+            s16x4 xP[NQ], xQ[NQ];
+            bool uP = cP >= 0, uQ = false;
+            ld_x(cP, xP);
+            cP = ld_col(2);
+            for (int it = 0; it < nmax; it += 2) {
+                uQ = cQ >= 0 && it + 1 < nmax;
+                ld_x(cQ, xQ);
+                cQ = ld_col(it + 3);
+                mma(it, aP, xP, uP);
+                aP = ld_a(it + 2);
+                uP = cP >= 0 && it + 2 < nmax;
+                ld_x(cP, xP);
+                cP = ld_col(it + 4);
+                mma(it + 1, aQ, xQ, uQ);
+                aQ = ld_a(it + 3);
+            }
+        }
+    } else {
+        fetch(0, a_cur, b_cur, use_cur);
+        for (int it = 0; it < nmax; ++it) {
+            fetch(it + 1, a_nxt, b_nxt, use_nxt);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc[q] = mfma4<BF16>(a_cur, b_cur[q], acc[q]);
+            a_cur = a_nxt;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) b_cur[q] = b_nxt[q];
+            if constexpr (LONG) {
+                if (it % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
+            }
         }
     }
     if constexpr (LONG) carry();
@@ -146,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     if constexpr (LONG) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const int xc = q * 4 + t;
+            const int xc = TILED ? t * NQ + q : q * 4 + t;
             if (xc >= k) continue;
             float* yp = Y + (size_t)xc * ldy + r0;
 #pragma unroll
@@ -157,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int xc = q * 4 + t;
+        const int xc = TILED ? t * NQ + q : q * 4 + t;
         if (xc >= k) continue;
         float* yp = Y + (size_t)xc * ldy + r0;
         if (r0 + 3 < rows) {
@@ -178,12 +246,26 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
                      float* Y, int ldy) {
     const long long slices = (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice;
     const long long blocks = (slices * kWave + kBlock - 1) / kBlock;
-    if (long_rows)
-        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, true, TILED>), dim3((unsigned)blocks), dim3(kBlock), 0, s,
-                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
-    else
-        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, false, TILED>), dim3((unsigned)blocks), dim3(kBlock), 0, s,
-                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    // 16-byte tile loads: every tile pair of a chunk 16-byte aligned (an even
+    // stride and chunk size; chunks start at multiples of 16) and two of
+    // them to load.
+    constexpr bool kX16 = TILED && NQ >= 2;
+    const bool x16 = kX16 && ldx % 2 == 0 && k % 2 == 0 && ((uintptr_t)X & 15) == 0;
+    if (long_rows) {
+        if (x16)
+            hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, true, TILED, kX16>), dim3((unsigned)blocks), dim3(kBlock), 0,
+                               s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+        else
+            hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, true, TILED, false>), dim3((unsigned)blocks), dim3(kBlock),
+                               0, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    } else {
+        if (x16)
+            hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, false, TILED, kX16>), dim3((unsigned)blocks), dim3(kBlock),
+                               0, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+        else
+            hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, false, TILED, false>), dim3((unsigned)blocks), dim3(kBlock),
+                               0, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    }
     return hipGetLastError();
 }
 
@@ -191,9 +273,13 @@ template <bool BF16, bool TILED>
 hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                     const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
                     float* Y, int ldy) {
-    // Up to 32 right-hand sides per launch (8 quads of accumulators).
-    for (int c = 0; c < k; c += 32) {
-        const int kc = std::min(32, k - c);
+    // Up to 32 right-hand sides per launch (8 quads of accumulators); 16
+    // with tiled X, whose two register sets of 8 quads would cost the
+    // kernel half its waves (156 registers against 52 for 4 quads), more
+    // than reading A once more per 16 right-hand sides.
+    constexpr int kChunk = TILED ? 16 : 32;
+    for (int c = 0; c < k; c += kChunk) {
+        const int kc = std::min(kChunk, k - c);
         // Right-hand side c: column c, or offset c inside every tile.
         const unsigned short* Xc = X + (TILED ? (size_t)c * 4 : (size_t)c * ldx);
         float* Yc = Y + (size_t)c * ldy;
@@ -205,8 +291,10 @@ hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols
             e = launch_nq<BF16, 2, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
         else if (nq <= 4)
             e = launch_nq<BF16, 4, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
-        else
+        else if constexpr (!TILED)
             e = launch_nq<BF16, 8, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+        else
+            e = hipErrorInvalidValue;  // not reached: tiled chunks hold at most 4 quads
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
