@@ -449,7 +449,7 @@ def main():
     gflops = 2.0 * A.nnz / (ms_per_step * 1e-3) / 1e9
 
     parts, graph_ms = rt.profile(max(10, min(100, args.steps)))
-    overlap = rt.launch_modes()["overlap"]
+    modes = rt.launch_modes()
     for p, (name, s) in zip(parts, ck.parts()):
         if name in ("bcsr",):
             p["block_index_bytes"] = 4 * s.b4csr_block_num + 4 * (s.b4csr_row_num + 1)
@@ -488,7 +488,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "rows": A.height, "cols": A.width, "nnz": A.nnz,
                    "partition": summary, "tables": os.path.relpath(tables, ROOT),
-                   "launch": "graph" if graph else "direct", "overlap": overlap, "tuned": not args.no_tune},
+                   "launch": "graph" if graph else "direct", "overlap": modes["overlap"],
+                   "dia_in_first_pass": modes["fuse"], "tuned": not args.no_tune},
         "hbm_gbps_algorithmic": round(total_bytes / (ms_per_step * 1e-3) / 1e9, 2),
         "roofline": {
             "bound": "hbm",

@@ -303,7 +303,8 @@ if hasattr(lib, "clspmv_init_mat_kernels"):
     _sig("clspmv_cocktail_tune", C.c_int, VP, C.c_int, P(C.c_int))
     _sig("clspmv_cocktail_set_graph", C.c_int, VP, C.c_int)
     _sig("clspmv_cocktail_set_overlap", C.c_int, VP, C.c_int)
-    _sig("clspmv_cocktail_launch_modes", C.c_int, VP, P(C.c_int), P(C.c_int))
+    _sig("clspmv_cocktail_set_fuse", C.c_int, VP, C.c_int)
+    _sig("clspmv_cocktail_launch_modes", C.c_int, VP, P(C.c_int), P(C.c_int), P(C.c_int))
     _sig("clspmv_evaluate", C.c_int, P(CocktailMatrix), PF, PF, PF, P(EvalReport))
 
 
@@ -971,12 +972,19 @@ class CocktailRuntime:
         earlier parts (True, the default) or everything on one stream."""
         check(lib.clspmv_cocktail_set_overlap(self.k, int(overlap)), "cocktail_set_overlap")
 
+    def set_fuse(self, fuse: bool):
+        """Sum an FP32 DIA part directly ahead of an x-panel part in that
+        part's first-pass launch (True) or in its own launch (False, the
+        default after init)."""
+        check(lib.clspmv_cocktail_set_fuse(self.k, int(fuse)), "cocktail_set_fuse")
+
     def launch_modes(self) -> dict:
-        """{"graph": bool, "overlap": bool}; overlap is True only when a pass
-        actually splits across the two streams."""
-        g, o = C.c_int(), C.c_int()
-        check(lib.clspmv_cocktail_launch_modes(self.k, C.byref(g), C.byref(o)), "cocktail_launch_modes")
-        return {"graph": bool(g.value), "overlap": bool(o.value)}
+        """{"graph", "overlap", "fuse"}: overlap / fuse are True only when a
+        pass actually uses them."""
+        g, o, f = C.c_int(), C.c_int(), C.c_int()
+        check(lib.clspmv_cocktail_launch_modes(self.k, C.byref(g), C.byref(o), C.byref(f)),
+              "cocktail_launch_modes")
+        return {"graph": bool(g.value), "overlap": bool(o.value), "fuse": bool(f.value)}
 
     def close(self):
         if self.k:

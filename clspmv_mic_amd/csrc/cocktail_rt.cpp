@@ -11,7 +11,10 @@
 // encoding's first pass (CSR/COO parts) reads only x and writes the part's
 // own segment sums, so it runs on a second stream beside the earlier parts'
 // kernels (the DIA band of a power-law matrix, say); the second pass, which
-// adds into y, waits for it and keeps its place in the order.
+// adds into y, waits for it and keeps its place in the order.  Or (fuse) an
+// FP32 DIA part directly ahead of such a part rides along in that part's
+// first-pass launch as extra work-groups, which fill the first pass's
+// ragged end; its update still reaches y before the second pass's.
 
 #include <algorithm>
 #include <cmath>
@@ -41,6 +44,7 @@ struct clspmv_cocktail_kernels {
     // Panel first passes on `side`, forked from and joined back into the
     // launching stream by the two events; false: one stream throughout.
     bool overlap = true;
+    bool fuse = false;  // DIA part summed in the next part's first-pass launch
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
 };
@@ -75,11 +79,33 @@ int overlap_from(const clspmv_cocktail_gpu* g) {
     return -1;
 }
 
-bool overlaps(const clspmv_cocktail_kernels* k) { return k->overlap && k->side && overlap_from(k->gpu) >= 0; }
+// Index of an FP32 DIA part directly followed by a part with a first pass,
+// or -1.
+int fuse_at(const clspmv_cocktail_gpu* g) {
+    for (size_t i = 0; i + 1 < g->parts.size(); ++i)
+        if (dmat_dia_tail_ok(g->parts[i]) && dmat_has_prepass(g->parts[i + 1])) return (int)i;
+    return -1;
+}
+
+bool fuses(const clspmv_cocktail_kernels* k) { return k->fuse && fuse_at(k->gpu) >= 0; }
+bool overlaps(const clspmv_cocktail_kernels* k) {
+    return !fuses(k) && k->overlap && k->side && overlap_from(k->gpu) >= 0;
+}
 
 // One pass over every part into y, on stream s (and k->side).
 int launch_all(hipStream_t s, const clspmv_cocktail_kernels* k) {
     const clspmv_cocktail_gpu* g = k->gpu;
+    if (fuses(k)) {
+        const int d = fuse_at(g);
+        for (size_t i = 0; i < g->parts.size(); ++i) {
+            if ((int)i == d) continue;  // summed by the next part's first-pass launch
+            int st = (int)i == d + 1 ? dmat_prepass(s, g->parts[i], g->vec, g->parts[d], g->res)
+                                     : dmat_launch(s, g->parts[i], g->methods[i], g->vec, g->res);
+            if (!st && (int)i == d + 1) st = dmat_launch_rest(s, g->parts[i], g->methods[i], g->vec, g->res);
+            if (st) return st;
+        }
+        return CLSPMV_OK;
+    }
     const int first = overlaps(k) ? overlap_from(g) : -1;
     if (first < 0) {
         for (size_t i = 0; i < g->parts.size(); ++i) {
@@ -520,10 +546,14 @@ int clspmv_cocktail_tune(clspmv_cocktail_kernels* k, int ntimes, int* graph) {
         return CLSPMV_OK;
     };
     double best_ms = 1e30;
-    bool best_graph = true, best_overlap = true;
-    const int n_overlap = overlap_from(g) >= 0 ? 2 : 1;
-    for (int o = 0; o < n_overlap && !st; ++o) {
-        k->overlap = o == 0;
+    bool best_graph = true, best_overlap = true, best_fuse = false;
+    // (overlap, fuse) candidates: the first is the default.
+    std::vector<std::pair<bool, bool>> modes{{true, false}};
+    if (overlap_from(g) >= 0) modes.push_back({false, false});
+    if (fuse_at(g) >= 0) modes.push_back({false, true});
+    for (size_t m = 0; m < modes.size() && !st; ++m) {
+        k->overlap = modes[m].first;
+        k->fuse = modes[m].second;
         double t_graph = 0.0, t_direct = 0.0;
         if (!(st = capture(k)))
             st = best_of3([&](hipStream_t q) -> int {
@@ -532,11 +562,12 @@ int clspmv_cocktail_tune(clspmv_cocktail_kernels* k, int ntimes, int* graph) {
             }, &t_graph);
         if (!st) st = best_of3([&](hipStream_t q) { return launch_all(q, k); }, &t_direct);
         if (st) break;
-        if (t_graph < best_ms) best_ms = t_graph, best_graph = true, best_overlap = k->overlap;
-        if (t_direct < best_ms) best_ms = t_direct, best_graph = false, best_overlap = k->overlap;
+        if (t_graph < best_ms) best_ms = t_graph, best_graph = true, best_overlap = k->overlap, best_fuse = k->fuse;
+        if (t_direct < best_ms) best_ms = t_direct, best_graph = false, best_overlap = k->overlap, best_fuse = k->fuse;
     }
-    if (!st && k->overlap != best_overlap) {
+    if (!st && (k->overlap != best_overlap || k->fuse != best_fuse)) {
         k->overlap = best_overlap;
+        k->fuse = best_fuse;
         st = capture(k);
     }
     e = hipMemcpyAsync(g->res, saved, ybytes, hipMemcpyDeviceToDevice, s);
@@ -565,10 +596,20 @@ int clspmv_cocktail_set_overlap(clspmv_cocktail_kernels* k, int overlap) {
     return capture(k);
 }
 
-int clspmv_cocktail_launch_modes(const clspmv_cocktail_kernels* k, int* graph, int* overlap) {
+int clspmv_cocktail_set_fuse(clspmv_cocktail_kernels* k, int fuse) {
+    if (!k || !k->exec) return fail(CLSPMV_ERR_INVALID, "kernels not initialised");
+    int st = use_ctx_device(k->ctx);
+    if (st) return st;
+    if (k->fuse == (fuse != 0)) return CLSPMV_OK;
+    k->fuse = fuse != 0;
+    return capture(k);
+}
+
+int clspmv_cocktail_launch_modes(const clspmv_cocktail_kernels* k, int* graph, int* overlap, int* fuse) {
     if (!k || !k->gpu) return fail(CLSPMV_ERR_INVALID, "kernels not initialised");
     if (graph) *graph = k->use_graph ? 1 : 0;
     if (overlap) *overlap = overlaps(k) ? 1 : 0;
+    if (fuse) *fuse = fuses(k) ? 1 : 0;
     return CLSPMV_OK;
 }
 

@@ -194,12 +194,72 @@ __device__ __forceinline__ void seg_store(const SegSums& o, int lane, float* __r
     if (lane == kWave - 1) store_at(out, b + nf - 1, o.last);
 }
 
+__device__ __forceinline__ float x_at(const float* __restrict__ x, int c, int cols) {
+    return (unsigned)c < (unsigned)cols ? x[c] : 0.0f;
+}
+
+// The DIA rows of work-group b of the tail: four rows per thread (strided
+// by the work-group), per row four partial sums over the diagonals (the
+// remainder in the first) added in order -- dia_kernel method 1's
+// arithmetic.  Loads of rows past the end are clamped; those rows are not
+// written.
+template <int NT>
+__device__ __forceinline__ void dia_tail_rows(const launch::DiaTail& d, const float* __restrict__ x, int b) {
+    constexpr int R = launch::kDiaTailRows / NT, U = 4;
+    const int r0 = b * launch::kDiaTailRows + (int)threadIdx.x;
+    int rr[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) rr[k] = min(r0 + k * NT, d.rows - 1);
+    float a[R][U];
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[k][u] = 0.0f;
+    int j = 0;
+    for (; j + U <= d.ndiag; j += U) {
+        float dv[R][U], xv[R][U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int o = d.off[j + u];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                dv[k][u] = d.val[(size_t)(j + u) * d.len + rr[k]];
+                xv[k][u] = x_at(x, rr[k] + o, d.cols);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+#pragma unroll
+            for (int u = 0; u < U; ++u) a[k][u] = fmaf(dv[k][u], xv[k][u], a[k][u]);
+    }
+    for (; j < d.ndiag; ++j) {
+        const int o = d.off[j];
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            a[k][0] = fmaf(d.val[(size_t)j * d.len + rr[k]], x_at(x, rr[k] + o, d.cols), a[k][0]);
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int r = r0 + k * NT;
+        if (r >= d.rows) continue;
+        float s = 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) s += a[k][u];
+        d.y[r] += s;
+    }
+}
+
 template <int NT, class V>
 __global__ __launch_bounds__(NT) void panel_pass1_kernel(
-    const int4* __restrict__ wg, int pw, int cols,
+    const int4* __restrict__ wg, int nwg, int pw, int cols,
     const unsigned short* __restrict__ pcol, const typename V::T* __restrict__ pval, int pad_at,
     const int* __restrict__ sc_seg, const int* __restrict__ sc_start,
-    const float* __restrict__ x, bool xa, float* __restrict__ seg_val) {
+    const float* __restrict__ x, bool xa, float* __restrict__ seg_val, launch::DiaTail tail) {
+    static_assert(launch::kDiaTailRows % NT == 0, "tail rows must split evenly over the threads");
+    if ((int)blockIdx.x >= nwg) {  // a work-group of the DIA tail (the whole group)
+        dia_tail_rows<NT>(tail, x, (int)blockIdx.x - nwg);
+        return;
+    }
     extern __shared__ float xs[];
     // The work-group's sub-chunk table, read once (loop indices then come
     // from LDS instead of one dependent memory load per sub-chunk).
@@ -209,7 +269,7 @@ __global__ __launch_bounds__(NT) void panel_pass1_kernel(
     // they run on one XCD, whose L2 then fetches that panel of x once
     // (block i goes to XCD i mod 8, so the plain order would have every
     // XCD stage every panel).
-    const int4 t = wg[xcd_tile(blockIdx.x, gridDim.x)];
+    const int4 t = wg[xcd_tile(blockIdx.x, nwg)];
     const int panel = t.x, sc0 = t.y, sc1 = t.z;
     const int xb = panel * pw;
     const int w = min(pw, cols - xb);
@@ -437,25 +497,34 @@ hipError_t panel_prepare() {
 
 hipError_t panel_pass1(hipStream_t s, int nwg, const int* wg, int pw, int cols, const unsigned short* pcol,
                        int dtype, const void* pval, int pad_at, const int* sc_seg, const int* sc_start,
-                       const float* x, bool x_aligned, float* seg_val) {
+                       const float* x, bool x_aligned, float* seg_val, const DiaTail* tail) {
     if (nwg <= 0) return hipSuccess;
     if (pw <= 0 || pw > kPanelMax) return hipErrorInvalidValue;
+    DiaTail dt{0, 0, 0, 0, nullptr, nullptr, nullptr};
+    int ntail = 0;
+    if (tail && tail->rows > 0 && tail->ndiag > 0) {
+        dt = *tail;
+        ntail = (dt.rows + kDiaTailRows - 1) / kDiaTailRows;
+    }
+    const int grid = nwg + ntail;
     const size_t lds = sizeof(float) * ((size_t)pw + 1);  // x panel + the padding entries' 0
     const int4* wg4 = reinterpret_cast<const int4*>(wg);
     const unsigned short* p16 = static_cast<const unsigned short*>(pval);
     switch (dtype) {
         case 0:
-            hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValF32>), dim3(nwg), dim3(kPanelThreads), lds,
-                               s, wg4, pw, cols, pcol, static_cast<const float*>(pval), pad_at, sc_seg, sc_start, x,
-                               x_aligned, seg_val);
+            hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValF32>), dim3(grid), dim3(kPanelThreads), lds,
+                               s, wg4, nwg, pw, cols, pcol, static_cast<const float*>(pval), pad_at, sc_seg,
+                               sc_start, x, x_aligned, seg_val, dt);
             break;
         case 1:
-            hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValBF16>), dim3(nwg), dim3(kPanelThreads),
-                               lds, s, wg4, pw, cols, pcol, p16, pad_at, sc_seg, sc_start, x, x_aligned, seg_val);
+            hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValBF16>), dim3(grid), dim3(kPanelThreads),
+                               lds, s, wg4, nwg, pw, cols, pcol, p16, pad_at, sc_seg, sc_start, x, x_aligned,
+                               seg_val, dt);
             break;
         case 2:
-            hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValF16>), dim3(nwg), dim3(kPanelThreads),
-                               lds, s, wg4, pw, cols, pcol, p16, pad_at, sc_seg, sc_start, x, x_aligned, seg_val);
+            hipLaunchKernelGGL((panel_pass1_kernel<kPanelThreads, ValF16>), dim3(grid), dim3(kPanelThreads),
+                               lds, s, wg4, nwg, pw, cols, pcol, p16, pad_at, sc_seg, sc_start, x, x_aligned,
+                               seg_val, dt);
             break;
         default:
             return hipErrorInvalidValue;

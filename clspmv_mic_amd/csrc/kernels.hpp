@@ -71,10 +71,23 @@ static_assert(kLongSlots <= kLongRowF64 / kPanelSub, "every row longer than kLon
 // and adds the row sums into y.  Pass 1 touches nothing pass 2 of another
 // matrix or any other kernel writes, so it may run on a second stream while
 // earlier kernels update y, as long as pass 2 waits for it.
-// pad_at: entry index of 8 padding entries (column code kPanelPad, value 0).
+// The rows of an FP32 DIA part (y[r] += sum_d val[d * len + r] *
+// x[r + off[d]], r < rows) can ride along in a pass-1 launch: extra
+// work-groups after the panel ones take kDiaTailRows rows each, with
+// dia_kernel method 1's arithmetic per row (its result, bit for bit).
+// Pass 1 does not touch y, so the DIA update still lands before pass 2's.
+constexpr int kDiaTailRows = 4 * kPanelThreads;
+struct DiaTail {
+    int rows, cols, len, ndiag;
+    const int* off;
+    const float* val;
+    float* y;
+};
+// pad_at: entry index of 8 padding entries (column code kPanelPad, value 0);
+// tail: a DIA part to sum in the same launch, or null.
 hipError_t panel_pass1(hipStream_t s, int nwg, const int* wg, int pw, int cols, const unsigned short* pcol,
                        int dtype, const void* pval, int pad_at, const int* sc_seg, const int* sc_start,
-                       const float* x, bool x_aligned, float* seg_val);
+                       const float* x, bool x_aligned, float* seg_val, const DiaTail* tail = nullptr);
 hipError_t panel_pass2(hipStream_t s, const float* seg_val, int nblocks, const int* blk_row, const int* blk_seg,
                        const int* row_slot, const unsigned short* seg_dst, const int* blk_long,
                        const int* longs, float* y);

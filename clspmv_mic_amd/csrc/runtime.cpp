@@ -102,11 +102,24 @@ bool dmat_has_prepass(const clspmv_dmat* A) {
            I.cols > 0 && I.nnz > 0;
 }
 
-int dmat_prepass(hipStream_t s, const clspmv_dmat* A, const float* x) {
+bool dmat_dia_tail_ok(const clspmv_dmat* D) {
+    const clspmv_dmat_info& I = D->info;
+    return I.format == CLSPMV_FMT_DIA && D->dtype == CLSPMV_DT_F32 && I.rows > 0 && I.cols > 0 && I.nnz > 0 &&
+           D->ndiag > 0;
+}
+
+int dmat_prepass(hipStream_t s, const clspmv_dmat* A, const float* x, const clspmv_dmat* dia, float* y) {
     if (!dmat_has_prepass(A)) return CLSPMV_OK;
+    launch::DiaTail tail{0, 0, 0, 0, nullptr, nullptr, nullptr};
+    if (dia) {
+        if (!dmat_dia_tail_ok(dia) || !y) return fail(CLSPMV_ERR_INVALID, "DIA part not fit for the pass-1 tail");
+        tail = launch::DiaTail{std::min(dia->info.rows, dia->len), dia->info.cols, dia->len, dia->ndiag, dia->offs,
+                               dia->val, y};
+    }
     const hipError_t e = launch::panel_pass1(s, A->pnwg, A->pwg, A->pw, A->info.cols, A->pcol, A->dtype,
                                              A->dtype ? (const void*)A->plpval : (const void*)A->pval, A->ppad,
-                                             A->psc_seg, A->psc_start, x, aligned16(x), A->pseg);
+                                             A->psc_seg, A->psc_start, x, aligned16(x), A->pseg,
+                                             dia ? &tail : nullptr);
     if (e != hipSuccess)
         return fail(CLSPMV_ERR_HIP, "panel pass 1 failed (format %d): %s", A->info.format, hipGetErrorString(e));
     return CLSPMV_OK;

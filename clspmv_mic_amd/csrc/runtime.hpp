@@ -149,7 +149,11 @@ int dmat_launch(hipStream_t stream, const clspmv_dmat* A, int method,
 // other encodings), dmat_launch_rest the ones that update y, which must be
 // ordered after the prepass.  dmat_launch = prepass then rest.
 bool dmat_has_prepass(const clspmv_dmat* A);
-int dmat_prepass(hipStream_t stream, const clspmv_dmat* A, const float* x);
+// dia (optional, with y): an FP32 DIA part whose rows the prepass launch
+// also sums into y (dmat_dia_tail_ok), with DIA method 1's arithmetic.
+bool dmat_dia_tail_ok(const clspmv_dmat* D);
+int dmat_prepass(hipStream_t stream, const clspmv_dmat* A, const float* x, const clspmv_dmat* dia = nullptr,
+                 float* y = nullptr);
 int dmat_launch_rest(hipStream_t stream, const clspmv_dmat* A, int method,
                      const float* x, float* y);
 

@@ -164,8 +164,9 @@ int clspmv_cocktail_profile(clspmv_cocktail_kernels* kernels, int ntimes,
  * are uploaded in both their own and the x-panel encoding, and the losing
  * encoding is freed); then the captured
  * graph is re-built, and graph launches and plain stream launches of the
- * same pass are timed, each with and without the x-panel first passes on a
- * second stream (clspmv_cocktail_set_overlap); the fastest of the four is
+ * same pass are timed, each with the x-panel first passes on a second
+ * stream (clspmv_cocktail_set_overlap), on one stream, and with a DIA part
+ * fused into a first-pass launch (clspmv_cocktail_set_fuse); the fastest is
  * kept.  y is preserved.  Tuning picks
  * by timing, so two processes may settle on different methods whose
  * results differ by rounding; untuned kernels follow the tables' choice.
@@ -181,11 +182,18 @@ int clspmv_cocktail_set_graph(clspmv_cocktail_kernels* kernels, int graph);
  * clspmv_cocktail_tune times both and keeps the faster.  Re-builds the
  * graph. */
 int clspmv_cocktail_set_overlap(clspmv_cocktail_kernels* kernels, int overlap);
+/* Sets whether an FP32 DIA part directly followed by an x-panel part is
+ * summed by extra work-groups of that part's first-pass launch (filling the
+ * first pass's ragged end) instead of its own launch; off after init.  The
+ * DIA rows then get DIA method 1's arithmetic whatever the part's method,
+ * and still reach y before the x-panel part's.  clspmv_cocktail_tune times
+ * it with the other modes.  Re-builds the graph. */
+int clspmv_cocktail_set_fuse(clspmv_cocktail_kernels* kernels, int fuse);
 /* The launch modes in force: *graph = 1 for one graph launch per pass;
  * *overlap = 1 when a pass actually runs a first pass on the second stream
- * (overlap set, and an x-panel part follows another part).  Either pointer
- * may be null. */
-int clspmv_cocktail_launch_modes(const clspmv_cocktail_kernels* kernels, int* graph, int* overlap);
+ * (overlap set, and an x-panel part follows another part); *fuse = 1 when
+ * a DIA part rides in a first-pass launch.  Any pointer may be null. */
+int clspmv_cocktail_launch_modes(const clspmv_cocktail_kernels* kernels, int* graph, int* overlap, int* fuse);
 
 /* evaluate (src/spmv_cocktail.cpp:1431): runs one SpMV from res, compares
  * with coores (the caller's expected y, e.g. from a serial COO SpMV) using

@@ -159,3 +159,51 @@ def test_overlapped_first_pass_identical(oracle):
     y64_3 = y64 + 2 * (y64 - y0.astype(np.float64))
     err, worst = oracle.scaled_error(ref, y64_3, 3 * bound)
     assert err <= 3 * TOL, f"scaled error {err:.3e} at row {worst}"
+
+
+def test_dia_in_first_pass_launch(oracle):
+    """The DIA band summed by extra work-groups of the COO part's first-pass
+    launch: within the tolerance of the FP64 oracle, the same in graph and
+    plain launches, bit-identical to separate launches when the DIA part
+    runs method 1 (whose arithmetic the tail uses), and exact on dyadic
+    values."""
+    A = matrices.power_law(200000, seed=7)  # DIA (the band) + COO under the shipped tables
+    ck = cs.analyze_matrix(A.copy(), cs.TABLES_DIR)
+    x = matrices.uniform_vector(A.width, seed=41)
+    y0 = matrices.uniform_vector(A.height, seed=42)
+    rt = cs.CocktailRuntime(ck, x, y0)
+    parts, _ = rt.profile(2)
+    dia_method = [p["method"] for p in parts if p["format"] == "dia"][0]
+    assert not rt.launch_modes()["fuse"]  # off after init
+    outs = {}
+    for fuse in (False, True):
+        rt.set_fuse(fuse)
+        assert rt.launch_modes()["fuse"] == fuse
+        for graph in (True, False):
+            rt.set_graph(graph)
+            rt.set_y(y0)
+            rt.do_spmv(2)
+            outs[(fuse, graph)] = rt.get_y()
+    rt.close()
+    np.testing.assert_array_equal(outs[(True, True)], outs[(True, False)])
+    np.testing.assert_array_equal(outs[(False, True)], outs[(False, False)])
+    if dia_method == 1:
+        np.testing.assert_array_equal(outs[(True, True)], outs[(False, True)])
+    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+    y64_2 = 2 * y64 - y0.astype(np.float64)
+    for key, y in outs.items():
+        err, worst = oracle.scaled_error(y, y64_2, 2 * bound)
+        assert err <= 2 * TOL, f"fuse, graph = {key}: scaled error {err:.3e} at row {worst}"
+    # Dyadic values: every partial sum exact, whatever the order.
+    D = _dyadic_copy(A)
+    ckd = cs.analyze_matrix(D.copy(), cs.TABLES_DIR)
+    xd = matrices.dyadic_vector(D.width, seed=43)
+    yd = matrices.dyadic_vector(D.height, seed=44)
+    rt = cs.CocktailRuntime(ckd, xd, yd)
+    rt.set_fuse(True)
+    rt.do_spmv(1)
+    got = rt.get_y()
+    modes = rt.launch_modes()
+    rt.close()
+    y64, _ = oracle.spmv_f64(D.row, D.col, D.val, xd, D.height, yd)
+    np.testing.assert_array_equal(got, y64.astype(np.float32), err_msg=str(modes))
