@@ -73,9 +73,9 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
 // never stored, and an MFMA output column depends only on its own B column.
 template <bool BF16, int NQ, bool LONG, bool TILED, bool X16>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
-    int brows, int rows, int cols, int k, const int* __restrict__ rp,
+    int brows, int rows, int cols, int kall, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
-    const unsigned short* __restrict__ X, int ldx, float* __restrict__ Y, int ldy) {
+    const unsigned short* __restrict__ Xall, int ldx, float* __restrict__ Yall, int ldy) {
     static_assert(!X16 || (TILED && NQ >= 2), "16-byte loads take two tiles of the tiled layout");
     static_assert(launch::kSpmmSlice * 4 == kWave, "a wave holds one slice, four lanes per block row");
     const int lane = threadIdx.x & (kWave - 1);
@@ -93,150 +93,162 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     // slice's width, padding slots feeding zeros.
     const int base = __builtin_amdgcn_readfirstlane(rp[sl]);
     const int nmax = __builtin_amdgcn_readfirstlane((rp[sl + 1] - base) / launch::kSpmmSlice);
-    f32x4 acc[NQ];
+    // One pass over the slice for right-hand sides [0, k) of X and Y.
+    auto run = [&](const unsigned short* __restrict__ X, float* __restrict__ Y, int k) {
+        f32x4 acc[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // Column-major X: operands of step `it` (zeros for padding slots and
-    // past the width).
-    auto fetch = [&](int it, s16x4& a, s16x4* b, bool& use) {
-        a = s16x4{0, 0, 0, 0};
-        int c0 = -1;
-        if (it < nmax) {
-            const size_t j = (size_t)base + (size_t)it * launch::kSpmmSlice + g;
-            a = *reinterpret_cast<const s16x4*>(val + j * 16 + t * 4);  // padding holds zeros
-            c0 = bcol[j];
+        for (int q = 0; q < NQ; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        // Column-major X: operands of step `it` (zeros for padding slots and
+        // past the width).
+        auto fetch = [&](int it, s16x4& a, s16x4* b, bool& use) {
+            a = s16x4{0, 0, 0, 0};
+            int c0 = -1;
+            if (it < nmax) {
+                const size_t j = (size_t)base + (size_t)it * launch::kSpmmSlice + g;
+                a = *reinterpret_cast<const s16x4*>(val + j * 16 + t * 4);  // padding holds zeros
+                c0 = bcol[j];
+            }
+            use = c0 >= 0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int xc = q * 4 + t;
+                b[q] = s16x4{0, 0, 0, 0};
+                if (use && xc < k) b[q] = ld4(X + (size_t)xc * ldx + c0, cols - c0);
+            }
+        };
+        // Software pipeline: block it + 1's loads are in flight while block it
+        // is multiplied.
+        s16x4 a_cur, a_nxt, b_cur[NQ], b_nxt[NQ];
+        bool use_cur, use_nxt;
+        double acc64[NQ][4];  // used by LONG only
+        if constexpr (LONG) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc64[q][i] = 0.0;
         }
-        use = c0 >= 0;
+        auto carry = [&]() {  // FP32 chunk sums into FP64 (LONG only)
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int xc = q * 4 + t;
-            b[q] = s16x4{0, 0, 0, 0};
-            if (use && xc < k) b[q] = ld4(X + (size_t)xc * ldx + c0, cols - c0);
-        }
-    };
-    // Software pipeline: block it + 1's loads are in flight while block it
-    // is multiplied.
-    s16x4 a_cur, a_nxt, b_cur[NQ], b_nxt[NQ];
-    bool use_cur, use_nxt;
-    double acc64[NQ][4];  // used by LONG only
-    if constexpr (LONG) {
+            for (int q = 0; q < NQ; ++q) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
+                for (int i = 0; i < 4; ++i) acc64[q][i] += (double)acc[q][i];
+                acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+        };
+        if constexpr (TILED) {
+            // Two register sets, P for even steps and Q for odd ones, take
+            // turns with nothing copied between them: a step's block column is
+            // loaded two steps ahead, its A values and X tiles one step ahead.
+            // Every load is unconditional (steps past the end re-read the last
+            // slot and are never multiplied), so each wait names an exact count
+            // and the next step's loads stay in flight during the MFMAs.
+            if (nmax > 0) {
+                auto slot = [&](int s) {
+                    return (size_t)base + (size_t)min(s, nmax - 1) * launch::kSpmmSlice + g;
+                };
+                auto ld_col = [&](int s) { return bcol[slot(s)]; };
+                auto ld_a = [&](int s) { return *reinterpret_cast<const s16x4*>(val + slot(s) * 16 + t * 4); };
+                auto ld_x = [&](int c0, s16x4* b) {
+                    // Padding slots (c0 < 0) read tile 0 and are zeroed in mma.
+                    const unsigned short* xt = X + (size_t)(c0 >= 0 ? c0 >> 2 : 0) * ldx * 4;
+                    if constexpr (X16) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) acc64[q][i] = 0.0;
-    }
-    auto carry = [&]() {  // FP32 chunk sums into FP64 (LONG only)
+                        for (int p = 0; p < NQ / 2; ++p) {
+                            const int r = min(t * NQ + 2 * p, k - 2);
+                            const s16x8 v = *reinterpret_cast<const s16x8*>(xt + (size_t)r * 4);
+                            b[2 * p] = v.lo;
+                            b[2 * p + 1] = v.hi;
+                        }
+                    } else {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc64[q][i] += (double)acc[q][i];
-            acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        }
-    };
-    if constexpr (TILED) {
-        // Two register sets, P for even steps and Q for odd ones, take
-        // turns with nothing copied between them: a step's block column is
-        // loaded two steps ahead, its A values and X tiles one step ahead.
-        // Every load is unconditional (steps past the end re-read the last
-        // slot and are never multiplied), so each wait names an exact count
-        // and the next step's loads stay in flight during the MFMAs.
-        if (nmax > 0) {
-            auto slot = [&](int s) {
-                return (size_t)base + (size_t)min(s, nmax - 1) * launch::kSpmmSlice + g;
-            };
-            auto ld_col = [&](int s) { return bcol[slot(s)]; };
-            auto ld_a = [&](int s) { return *reinterpret_cast<const s16x4*>(val + slot(s) * 16 + t * 4); };
-            auto ld_x = [&](int c0, s16x4* b) {
-                // Padding slots (c0 < 0) read tile 0 and are zeroed in mma.
-                const unsigned short* xt = X + (size_t)(c0 >= 0 ? c0 >> 2 : 0) * ldx * 4;
-                if constexpr (X16) {
-#pragma unroll
-                    for (int p = 0; p < NQ / 2; ++p) {
-                        const int r = min(t * NQ + 2 * p, k - 2);
-                        const s16x8 v = *reinterpret_cast<const s16x8*>(xt + (size_t)r * 4);
-                        b[2 * p] = v.lo;
-                        b[2 * p + 1] = v.hi;
+                        for (int q = 0; q < NQ; ++q)
+                            b[q] = *reinterpret_cast<const s16x4*>(xt + (size_t)min(t * NQ + q, k - 1) * 4);
                     }
-                } else {
+                };
+                auto mma = [&](int s, const s16x4& a, const s16x4* b, bool use) {
+                    const s16x4 z = s16x4{0, 0, 0, 0}, au = use ? a : z;
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q)
-                        b[q] = *reinterpret_cast<const s16x4*>(xt + (size_t)min(t * NQ + q, k - 1) * 4);
-                }
-            };
-            auto mma = [&](int s, const s16x4& a, const s16x4* b, bool use) {
-                const s16x4 z = s16x4{0, 0, 0, 0}, au = use ? a : z;
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) acc[q] = mfma4<BF16>(au, use ? b[q] : z, acc[q]);
-                if constexpr (LONG) {
-                    if (s % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
-                }
-            };
-            // Steps go in pairs with no exit inside a pair (a load used only
-            // past an exit would be moved down to its use): an odd last step
-            // pairs with one past the end, multiplied as zeros (which can
-            // turn a zero sum of -0 into +0; nothing else changes).
-            int cP = ld_col(0), cQ = ld_col(1);
-            s16x4 aP = ld_a(0), aQ = ld_a(1);
-            s16x4 xP[NQ], xQ[NQ];
-            bool uP = cP >= 0, uQ = false;
-            ld_x(cP, xP);
-            cP = ld_col(2);
-            for (int it = 0; it < nmax; it += 2) {
-                uQ = cQ >= 0 && it + 1 < nmax;
-                ld_x(cQ, xQ);
-                cQ = ld_col(it + 3);
-                mma(it, aP, xP, uP);
-                aP = ld_a(it + 2);
-                uP = cP >= 0 && it + 2 < nmax;
+                    for (int q = 0; q < NQ; ++q) acc[q] = mfma4<BF16>(au, use ? b[q] : z, acc[q]);
+                    if constexpr (LONG) {
+                        if (s % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
+                    }
+                };
+                // Steps go in pairs with no exit inside a pair (a load used only
+                // past an exit would be moved down to its use): an odd last step
+                // pairs with one past the end, multiplied as zeros (which can
+                // turn a zero sum of -0 into +0; nothing else changes).
+                int cP = ld_col(0), cQ = ld_col(1);
+                s16x4 aP = ld_a(0), aQ = ld_a(1);
+                s16x4 xP[NQ], xQ[NQ];
+                bool uP = cP >= 0, uQ = false;
                 ld_x(cP, xP);
-                cP = ld_col(it + 4);
-                mma(it + 1, aQ, xQ, uQ);
-                aQ = ld_a(it + 3);
+                cP = ld_col(2);
+                for (int it = 0; it < nmax; it += 2) {
+                    uQ = cQ >= 0 && it + 1 < nmax;
+                    ld_x(cQ, xQ);
+                    cQ = ld_col(it + 3);
+                    mma(it, aP, xP, uP);
+                    aP = ld_a(it + 2);
+                    uP = cP >= 0 && it + 2 < nmax;
+                    ld_x(cP, xP);
+                    cP = ld_col(it + 4);
+                    mma(it + 1, aQ, xQ, uQ);
+                    aQ = ld_a(it + 3);
+                }
+            }
+        } else {
+            fetch(0, a_cur, b_cur, use_cur);
+            for (int it = 0; it < nmax; ++it) {
+                fetch(it + 1, a_nxt, b_nxt, use_nxt);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) acc[q] = mfma4<BF16>(a_cur, b_cur[q], acc[q]);
+                a_cur = a_nxt;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) b_cur[q] = b_nxt[q];
+                if constexpr (LONG) {
+                    if (it % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
+                }
             }
         }
-    } else {
-        fetch(0, a_cur, b_cur, use_cur);
-        for (int it = 0; it < nmax; ++it) {
-            fetch(it + 1, a_nxt, b_nxt, use_nxt);
+        if constexpr (LONG) carry();
+        if (!active) return;
+        const int r0 = br * 4;
+        if constexpr (LONG) {
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) acc[q] = mfma4<BF16>(a_cur, b_cur[q], acc[q]);
-            a_cur = a_nxt;
+            for (int q = 0; q < NQ; ++q) {
+                const int xc = TILED ? t * NQ + q : q * 4 + t;
+                if (xc >= k) continue;
+                float* yp = Y + (size_t)xc * ldy + r0;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) b_cur[q] = b_nxt[q];
-            if constexpr (LONG) {
-                if (it % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
+                for (int i = 0; i < 4; ++i)
+                    if (r0 + i < rows) yp[i] = (float)((double)yp[i] + acc64[q][i]);
             }
+            return;
         }
-    }
-    if constexpr (LONG) carry();
-    if (!active) return;
-    const int r0 = br * 4;
-    if constexpr (LONG) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int xc = TILED ? t * NQ + q : q * 4 + t;
             if (xc >= k) continue;
             float* yp = Y + (size_t)xc * ldy + r0;
+            if (r0 + 3 < rows) {
+                f32x4 y = *reinterpret_cast<const f32x4*>(yp);
+                y += acc[q];
+                *reinterpret_cast<f32x4*>(yp) = y;
+            } else {
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (r0 + i < rows) yp[i] = (float)((double)yp[i] + acc64[q][i]);
+                for (int i = 0; i < 4; ++i)
+                    if (r0 + i < rows) yp[i] += acc[q][i];
+            }
         }
-        return;
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int xc = TILED ? t * NQ + q : q * 4 + t;
-        if (xc >= k) continue;
-        float* yp = Y + (size_t)xc * ldy + r0;
-        if (r0 + 3 < rows) {
-            f32x4 y = *reinterpret_cast<const f32x4*>(yp);
-            y += acc[q];
-            *reinterpret_cast<f32x4*>(yp) = y;
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (r0 + i < rows) yp[i] += acc[q][i];
-        }
+    };
+    if constexpr (TILED) {
+        // 4 * NQ right-hand sides at a time over this slice: the later
+        // passes find the slice's blocks and column indices in cache, so A
+        // is read from memory about once whatever k is.
+        for (int c = 0; c < kall; c += 4 * NQ)
+            run(Xall + (size_t)c * 4, Yall + (size_t)c * ldy, min(4 * NQ, kall - c));
+    } else {
+        run(Xall, Yall, kall);
     }
 }
 
@@ -247,8 +259,9 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
     const long long slices = (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice;
     const long long blocks = (slices * kWave + kBlock - 1) / kBlock;
     // 16-byte tile loads: every tile pair of a chunk 16-byte aligned (an even
-    // stride and chunk size; chunks start at multiples of 16) and two of
-    // them to load.
+    // stride and chunk size -- chunks are 16 right-hand sides but the last,
+    // k - 16m, even with k -- and chunks start at multiples of 16) and two
+    // of them to load.
     constexpr bool kX16 = TILED && NQ >= 2;
     const bool x16 = kX16 && ldx % 2 == 0 && k % 2 == 0 && ((uintptr_t)X & 15) == 0;
     if (long_rows) {
@@ -273,17 +286,17 @@ template <bool BF16, bool TILED>
 hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                     const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
                     float* Y, int ldy) {
-    // Up to 32 right-hand sides per launch (8 quads of accumulators); 16
-    // with tiled X, whose two register sets of 8 quads would cost the
-    // kernel half its waves (156 registers against 52 for 4 quads), more
-    // than reading A once more per 16 right-hand sides.
-    constexpr int kChunk = TILED ? 16 : 32;
+    // Up to 32 right-hand sides per launch (8 quads of accumulators).  With
+    // tiled X one launch takes every right-hand side, 16 at a time inside
+    // each wave: two register sets of 8 quads would cost the kernel half its
+    // waves (156 registers against 52 for 4 quads).
+    constexpr int kChunk = TILED ? 1 << 30 : 32;
     for (int c = 0; c < k; c += kChunk) {
         const int kc = std::min(kChunk, k - c);
         // Right-hand side c: column c, or offset c inside every tile.
         const unsigned short* Xc = X + (TILED ? (size_t)c * 4 : (size_t)c * ldx);
         float* Yc = Y + (size_t)c * ldy;
-        const int nq = (kc + 3) / 4;
+        const int nq = (std::min(kc, TILED ? 16 : 32) + 3) / 4;
         hipError_t e;
         if (nq <= 1)
             e = launch_nq<BF16, 1, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
