@@ -179,6 +179,7 @@ int clspmv_cocktail_set_graph(clspmv_cocktail_kernels* kernels, int graph);
  * a second stream beside the parts launched before them, the default after
  * init; 0 = every kernel on one stream.  Either way each row of y receives
  * the parts' contributions in the same order, so results are identical.
+ * Ignored while a DIA part is fused (clspmv_cocktail_set_fuse).
  * clspmv_cocktail_tune times both and keeps the faster.  Re-builds the
  * graph. */
 int clspmv_cocktail_set_overlap(clspmv_cocktail_kernels* kernels, int overlap);
