@@ -64,7 +64,8 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
     // row's segments by its per-panel runs (a run of L > 1 entries becomes
     // at most L/kSub + 2 segments), at most `cap` segments (a longer row
     // alone, up to kBlockSegsMax).  Two pass-2 work-groups run per CU; cap
-    // splits the segments evenly over whole rounds of them.
+    // splits the segments evenly over whole rounds of them (for one round,
+    // the smallest cap whose blocks fit it).
     if (rows >= (1 << 30)) return fail(CLSPMV_ERR_UNSUPPORTED, "panel encoding needs rows < 2^30");
     std::vector<int> rbound(rows);
     long long total_bound = 0;
@@ -88,27 +89,44 @@ int build_panel(clspmv_dmat* d, int rows, int cols, long long nnz, const int* ro
         }
         if (i != nnz) return fail(CLSPMV_ERR_INVALID, "panel encoding needs row-sorted entries");
     }
-    long long cap = launch::kBlockSegsMax;
-    {
-        const long long slots = 2ll * ncu;
-        const long long rounds = std::max(1ll, (total_bound + slots * cap - 1) / (slots * cap));
-        // 10 % headroom: greedy packing leaves blocks short of the cap.
-        cap = std::min<long long>(cap, std::max(1024ll, total_bound * 11 / (10 * rounds * slots) + 1));
-    }
-    std::vector<int> blk_row(1, 0);
-    {
-        long long bound_sum = 0;
+    // Greedy cut of the rows into blocks under a segment cap.
+    auto cut_blocks = [&](long long cap, std::vector<int>* out) {
+        long long bound_sum = 0, nb = 1;
         int r0 = 0;
+        if (out) out->assign(1, 0);
         for (int r = 0; r < rows; ++r) {
             if (r > r0 && (r - r0 == launch::kBlockRowsMax || bound_sum + rbound[r] > cap)) {
-                blk_row.push_back(r);
+                if (out) out->push_back(r);
+                ++nb;
                 r0 = r;
                 bound_sum = 0;
             }
             bound_sum += rbound[r];
         }
-        blk_row.push_back(rows);
+        if (out) out->push_back(rows);
+        return nb;
+    };
+    long long cap = launch::kBlockSegsMax;
+    {
+        const long long slots = 2ll * ncu;
+        const long long rounds = std::max(1ll, (total_bound + slots * cap - 1) / (slots * cap));
+        if (rounds == 1 && cut_blocks(cap, nullptr) <= slots) {
+            // One round: the smallest cap (not under 1024) whose blocks still
+            // fit it, so the blocks -- and the two per CU -- come out even.
+            long long lo = 1024, hi = cap;
+            while (lo < hi) {
+                const long long mid = (lo + hi) / 2;
+                if (cut_blocks(mid, nullptr) <= slots) hi = mid;
+                else lo = mid + 1;
+            }
+            cap = std::min<long long>(cap, lo);
+        } else {
+            // 10 % headroom: greedy packing leaves blocks short of the cap.
+            cap = std::min<long long>(cap, std::max(1024ll, total_bound * 11 / (10 * rounds * slots) + 1));
+        }
     }
+    std::vector<int> blk_row;
+    cut_blocks(cap, &blk_row);
     const int nb = (int)blk_row.size() - 1;
     if ((long long)P * (nb + 1) > (1ll << 26))
         return fail(CLSPMV_ERR_UNSUPPORTED, "%d panels x %d row blocks: too many groups", P, nb);
