@@ -1,16 +1,22 @@
 """Benchmark of the clspmv MI355X engine: autotuned Cocktail SpMV.
 
-One step = one SpMV y += A x over all parts of the partitioned matrix (one
-HIP-graph launch).  The default workload is the 1,000,000-row power-law
-matrix (row lengths Zipf(1.8) clipped to [1, 5000], uniform random columns,
-N(0,1) values) united with a band at offsets {0, +-1, +-1000}; --workload
-selects the other configurations.  Matrix, x (uniform[-1,1]) and y are
-resident in HBM before timing starts.
+One step = one SpMV y += A x over all parts of the partitioned matrix.  The
+default workload is the 27-point stencil on a 128^3 grid (2,097,152 rows,
+55.7 M nonzeros, N(0,1) off-diagonals, diagonally dominant centre); the
+same JSON line carries the 1,000,000-row power-law matrix (row lengths
+Zipf(1.8) clipped to [1, 5000], uniform random columns, N(0,1) values,
+united with a band at offsets {0, +-1, +-1000}) under "powerlaw".
+--workload selects one configuration alone, or the SpMM, serving and
+per-format sweeps.  Matrix, x (uniform[-1,1]) and y are resident in HBM
+before timing starts.
 
 Prints one JSON line: GFLOP/s of the whole SpMV (2 * nnz / t), the roofline
-of the dominant kernel (algorithmic bytes / its event-timed duration against
-8 TB/s), and the reference's serial CSR SpMV timed on the host as the CPU
-baseline.
+of the dominant kernel (the bytes its format must move / its event-timed
+duration, against 8 TB/s; the byte count that also credits a column index
+per nonzero is kept beside it), the same pass with the caches scrubbed
+before every pass, and the CPU baseline on the same matrix: the
+reference's serial csr_spmv on one core and a row-parallel CSR SpMV over
+all host threads.
 """
 
 import argparse
@@ -66,47 +72,178 @@ def part_min_bytes(part, rows, cols):
     return 8 * nnz + 4 * (rows + 1) + vec
 
 
-def cpu_baseline(A, x, y_gpu, budget_s=10.0):
-    """The reference's serial csr_spmv (include/spmv_serial.h:199) compiled
-    from its sources (oracle/_ref), else the oracle's restatement, on the
-    whole matrix for as many iterations as fit in budget_s."""
-    import ctypes as C
+def cpu_baseline(A, x, y_gpu, budget_s=6.0):
+    """The CPU path timed on this host over the whole matrix: the
+    reference's serial csr_spmv (include/spmv_serial.h:199) compiled from
+    its sources (oracle/_ref; the oracle's restatement of it where that is
+    absent) on one core, and the same row loop over all host threads
+    (OpenMP, rows split statically) -- each for as many calls as fit in
+    budget_s.  The reported value is the faster (all-thread) leg."""
     import numpy as np
     import clspmv_mic_amd as cs
     import oracle
     csr = cs.coo2csr(A.copy())
     arrs = csr.arrays()
     y = np.zeros(A.height, np.float32)
+    orc = oracle.Oracle()
     kind = "port"
     if oracle.reference_available():
         ref = oracle.Reference()
-        run = lambda: ref.spmv("csr", csr.s, x, y)  # noqa: E731
+        serial = lambda: ref.spmv("csr", csr.s, x, y)  # noqa: E731
         kind = "reference"
     else:
-        orc = oracle.Oracle()
-        run = lambda: orc.csr_spmv(arrs["rp"], arrs["col"], arrs["val"], x, y)  # noqa: E731
-    t0 = time.perf_counter()
-    run()
-    t1 = time.perf_counter()
-    single = max(t1 - t0, 1e-6)
-    y_first = y.copy()
-    iters = max(2, min(200, int(budget_s / single)))
-    t0 = time.perf_counter()
-    for _ in range(iters):
+        serial = lambda: orc.csr_spmv(arrs["rp"], arrs["col"], arrs["val"], x, y)  # noqa: E731
+    threads = orc.max_threads()
+    parallel = lambda: orc.csr_spmv_threads(threads, arrs["rp"], arrs["col"], arrs["val"], x, y)  # noqa: E731
+
+    def timed(run):
+        t0 = time.perf_counter()
         run()
-    t1 = time.perf_counter()
-    per = (t1 - t0) / iters
+        single = max(time.perf_counter() - t0, 1e-6)
+        iters = max(2, min(500, int(budget_s / single)))
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            run()
+        return (time.perf_counter() - t0) / iters, iters
+
+    per1, it1 = timed(serial)
+    pern, itn = timed(parallel)
     # GPU result against the FP64 oracle, scaled by |A||x| per row.
-    orc = oracle.Oracle()
     y64, bound = orc.spmv_f64(A.row, A.col, A.val, x, A.height)
     err, _ = orc.scaled_error(y_gpu, y64, bound)
+    gf = lambda per: round(2.0 * A.nnz / per / 1e9, 4)  # noqa: E731
     return {
-        "value": round(2.0 * A.nnz / per / 1e9, 4),
+        "value": gf(pern),
         "unit": "GFLOP/s",
-        "cores": 1,
-        "kind": kind,
-        "sample": f"whole matrix ({A.nnz} nnz), {iters} serial csr_spmv calls, {per * 1e3:.2f} ms each",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"whole matrix ({A.nnz} nnz): {itn} CSR SpMVs over {threads} threads "
+                  f"({pern * 1e3:.2f} ms each)",
+        "single_core": {"value": gf(per1), "unit": "GFLOP/s", "cores": 1, "kind": kind,
+                        "sample": f"whole matrix: {it1} serial csr_spmv calls ({per1 * 1e3:.2f} ms each)"},
         "gpu_scaled_error_vs_fp64": err,
+    }
+
+
+def measured_traffic(workload, kernel_prefix):
+    """HBM bytes per launch of a kernel from the counter table committed in
+    profiles/traffic_counters.md (rows: | workload | kernel | FETCH bytes |
+    WRITE bytes | traffic bytes | ...), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == workload and cells[1].startswith(kernel_prefix):
+            try:
+                return int(cells[4].replace(",", ""))
+            except ValueError:
+                return None
+    return None
+
+
+# Kernel-name prefix of each format's dominant kernel (rocprofv3 names).
+KERNEL_OF = {"dia": "dia", "bdia": "bdia", "coo": "panel_pass", "csr": "panel_pass", "bcsr": "bcsr_kernel",
+             "bell": "bell_kernel", "sbell": "sbell_kernel", "ell": "ell", "sell": "sell_kernel"}
+
+
+def measure_cocktail(args, name):
+    """Builds, partitions, tunes and times one Cocktail SpMV workload;
+    returns its JSON object (without the run-level keys)."""
+    import clspmv_mic_amd as cs
+    from clspmv_mic_amd import matrices
+    t0 = time.time()
+    A, desc = build_matrix(name)
+    print(f"[bench] matrix {name}: {A.height}x{A.width} nnz={A.nnz} ({time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    tables = args.tables or cs.TABLES_DIR
+    t0 = time.time()
+    ck = cs.analyze_matrix(A.copy(), tables)
+    t_analyze = time.time() - t0
+    summary = ck.summary()
+    print(f"[bench] partition {summary} ({t_analyze:.1f}s)", file=sys.stderr, flush=True)
+    x = matrices.uniform_vector(A.width, seed=1)
+    rt = cs.CocktailRuntime(ck, x)
+    graph = True
+    if not args.no_tune:
+        t0 = time.time()
+        graph = rt.tune(20)
+        print(f"[bench] tuned methods, launch mode {'graph' if graph else 'direct'} ({time.time() - t0:.1f}s)",
+              file=sys.stderr, flush=True)
+
+    # One pass from y = 0 for the CPU cross-check, then warm-up and timing.
+    rt.do_spmv(1)
+    y_one = rt.get_y()
+    rt.do_spmv(args.warmup)
+    rt.sync()
+    t0 = time.perf_counter()
+    rt.do_spmv_async(args.steps)
+    rt.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    gflops = 2.0 * A.nnz / (ms_per_step * 1e-3) / 1e9
+
+    parts, graph_ms = rt.profile(max(10, min(100, args.steps)))
+    modes = rt.launch_modes()
+    cold_ms = rt.time_cold(max(5, min(50, args.steps // 4)), 1 << 30)
+    for p, (pname, s) in zip(parts, ck.parts()):
+        if pname in ("bcsr",):
+            p["block_index_bytes"] = 4 * s.b4csr_block_num + 4 * (s.b4csr_row_num + 1)
+        elif pname == "bell":
+            p["block_index_bytes"] = 4 * s.b4ell_row_num * s.b4ell_block_num
+        elif pname == "sbell":
+            p["block_index_bytes"] = 4 * (s.sbell_slice_ptr[s.sbell_slice_num] if s.sbell_slice_num else 0)
+    for p in parts:
+        p["bytes"] = part_bytes(p, A.height, A.width)
+        p["format_bytes"] = part_min_bytes(p, A.height, A.width)
+        p["gbps"] = p["format_bytes"] / (p["ms"] * 1e-3) / 1e9 if p["ms"] > 0 else 0.0
+    dom = max(parts, key=lambda p: p["ms"])
+    # The whole pass: every part's format bytes, x and y counted once.
+    vec = 4 * A.width + 8 * A.height
+    pass_format = sum(p["format_bytes"] - vec for p in parts) + vec
+    pass_survey = 8 * A.nnz + 4 * (A.height + 1) + vec
+    rt.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(A, x, y_one)
+    frac = lambda b, ms: round(b / (ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4) if ms > 0 else 0.0  # noqa: E731
+    return {
+        "metric": "SpMV GFLOP/s & HBM GB/s (% roofline); served requests/s + p99 latency",
+        "value": round(gflops, 3),
+        "unit": "GFLOP/s",
+        "ms_per_step": round(ms_per_step, 6),
+        "config": {"workload": desc, "rows": A.height, "cols": A.width, "nnz": A.nnz,
+                   "partition": summary, "tables": os.path.relpath(tables, ROOT),
+                   "launch": "graph" if graph else "direct", "overlap": modes["overlap"],
+                   "dia_in_first_pass": modes["fuse"], "tuned": not args.no_tune},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"{dom['format']} method {dom['method']}",
+            # Bytes the part's format must move per launch: true nonzeros'
+            # values (+ a column index per nonzero for CSR/COO, + block
+            # indices for block formats), x read once, y read and written.
+            "achieved": round(dom["gbps"], 2),
+            "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": frac(dom["format_bytes"], dom["ms"]),
+            "traffic": measured_traffic(name, KERNEL_OF.get(dom["format"], dom["format"])),
+            "algorithmic_bytes": dom["format_bytes"],
+            "kernel_ms": round(dom["ms"], 6),
+            # The count that credits 4 bytes of column index per nonzero to
+            # every format (DIA and block formats read none, so this frac
+            # can pass 1).
+            "index_credit_bytes": dom["bytes"],
+            "index_credit_frac": frac(dom["bytes"], dom["ms"]),
+        },
+        "pass_roofline": {"bytes": pass_format, "frac": frac(pass_format, ms_per_step),
+                          "index_credit_bytes": pass_survey, "index_credit_frac": frac(pass_survey, ms_per_step)},
+        "cold_cache": {"ms_per_step": round(cold_ms, 6),
+                       "value": round(2.0 * A.nnz / (cold_ms * 1e-3) / 1e9, 3),
+                       "pass_frac": frac(pass_format, cold_ms),
+                       "note": "each pass timed alone after a 512 MiB device copy (1 GiB moved) that "
+                               "evicts the L2s and the 256 MB memory-side cache"},
+        "parts": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in p.items()} for p in parts],
+        "graph_ms": round(graph_ms, 6),
+        "cpu_baseline": cpu,
     }
 
 
@@ -290,6 +427,7 @@ def run_serve(args):
     slo_rate = slo_load_served * cap
     # The same for the plain launches, whose requests wait for each other.
     slo_rate_plain = slo_load(1, plain) * plain
+    stats = server.stats()
     server.close()
     served = n / wall
     flops = 2.0 * float(np.sum(nnz_of[mat] * ks))
@@ -313,6 +451,7 @@ def run_serve(args):
                 "speedup_vs_plain": round(slo_rate / slo_rate_plain, 2) if slo_rate_plain > 0 else None,
                 "requests_per_probe": ns},
         "plain_latency_ms_p50": round(float(np.percentile(latb, 50)) * 1e3, 3),
+        "server_stats": stats,
         "gflops_served": round(flops / wall / 1e9, 2),
         "bf16_matrix_error_vs_fp64": {"matrices": len(errs16),
                                       "max": float(f"{max(errs16):.3e}") if errs16 else None,
@@ -381,8 +520,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="powerlaw",
-                    choices=["powerlaw", "stencil27", "lap2d", "block4x4", "spmm", "serve", "sweep"])
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "powerlaw", "stencil27", "lap2d", "block4x4", "spmm", "serve", "sweep"],
+                    help="headline: the 27-point stencil 128^3 with the 1M power-law matrix in the same line")
     ap.add_argument("--dtype", default="bf16", choices=["bf16", "fp16"], help="spmm: value type")
     ap.add_argument("--k", type=int, nargs="+", default=[8, 16, 32], help="spmm: right-hand sides")
     ap.add_argument("--err-columns", type=int, default=4, help="spmm: columns checked against FP64")
@@ -399,7 +539,6 @@ def main():
     ap.add_argument("--tables", default=None, help="directory holding benchmark/*.ben")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the tables' methods and graph launches")
-    ap.add_argument("--traffic", default=None, help="JSON of measured HBM bytes per launch by kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); nothing is measured on more "
@@ -413,105 +552,28 @@ def main():
     if args.workload == "sweep":
         return run_sweep(args)
 
-    import numpy as np
-    import clspmv_mic_amd as cs
-    from clspmv_mic_amd import matrices
-
-    t0 = time.time()
-    A, desc = build_matrix(args.workload)
-    print(f"[bench] matrix {args.workload}: {A.height}x{A.width} nnz={A.nnz} ({time.time() - t0:.1f}s)",
-          file=sys.stderr, flush=True)
-    tables = args.tables or cs.TABLES_DIR
-    t0 = time.time()
-    ck = cs.analyze_matrix(A.copy(), tables)
-    t_analyze = time.time() - t0
-    summary = ck.summary()
-    print(f"[bench] partition {summary} ({t_analyze:.1f}s)", file=sys.stderr, flush=True)
-    x = matrices.uniform_vector(A.width, seed=1)
-    rt = cs.CocktailRuntime(ck, x)
-    graph = True
-    if not args.no_tune:
-        t0 = time.time()
-        graph = rt.tune(20)
-        print(f"[bench] tuned methods, launch mode {'graph' if graph else 'direct'} ({time.time() - t0:.1f}s)",
-              file=sys.stderr, flush=True)
-
-    # One pass from y = 0 for the CPU cross-check, then warm-up and timing.
-    rt.do_spmv(1)
-    y_one = rt.get_y()
-    rt.do_spmv(args.warmup)
-    rt.sync()
-    t0 = time.perf_counter()
-    rt.do_spmv_async(args.steps)
-    rt.sync()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    gflops = 2.0 * A.nnz / (ms_per_step * 1e-3) / 1e9
-
-    parts, graph_ms = rt.profile(max(10, min(100, args.steps)))
-    modes = rt.launch_modes()
-    for p, (name, s) in zip(parts, ck.parts()):
-        if name in ("bcsr",):
-            p["block_index_bytes"] = 4 * s.b4csr_block_num + 4 * (s.b4csr_row_num + 1)
-        elif name == "bell":
-            p["block_index_bytes"] = 4 * s.b4ell_row_num * s.b4ell_block_num
-        elif name == "sbell":
-            p["block_index_bytes"] = 4 * (s.sbell_slice_ptr[s.sbell_slice_num] if s.sbell_slice_num else 0)
-    for p in parts:
-        p["bytes"] = part_bytes(p, A.height, A.width)
-        p["format_bytes"] = part_min_bytes(p, A.height, A.width)
-        p["gbps"] = p["bytes"] / (p["ms"] * 1e-3) / 1e9 if p["ms"] > 0 else 0.0
-    dom = max(parts, key=lambda p: p["ms"])
-    total_bytes = 8 * A.nnz + 4 * (A.height + 1) + 4 * A.width + 8 * A.height
-    traffic = None
-    if args.traffic and os.path.exists(args.traffic):
-        meas = json.load(open(args.traffic))
-        traffic = meas.get(dom["format"])
-    rt.close()
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(A, x, y_one)
-
+    names = ["stencil27", "powerlaw"] if args.workload == "headline" else [args.workload]
+    results = [measure_cocktail(args, n) for n in names]
     out = {
-        "metric": "SpMV GFLOP/s & HBM GB/s (% roofline); served requests/s + p99 latency",
-        "value": round(gflops, 3),
+        "metric": results[0]["metric"],
+        "value": results[0]["value"],
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 6),
+        "ms_per_step": results[0]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": desc, "rows": A.height, "cols": A.width, "nnz": A.nnz,
-                   "partition": summary, "tables": os.path.relpath(tables, ROOT),
-                   "launch": "graph" if graph else "direct", "overlap": modes["overlap"],
-                   "dia_in_first_pass": modes["fuse"], "tuned": not args.no_tune},
-        "hbm_gbps_algorithmic": round(total_bytes / (ms_per_step * 1e-3) / 1e9, 2),
-        "roofline": {
-            "bound": "hbm",
-            "kernel": f"{dom['format']} method {dom['method']}",
-            "achieved": round(dom["gbps"], 2),
-            "peak": PEAK_HBM_GBPS,
-            "unit": "GB/s",
-            "frac": round(dom["gbps"] / PEAK_HBM_GBPS, 4),
-            "traffic": traffic,
-            "algorithmic_bytes": dom["bytes"],
-            "kernel_ms": round(dom["ms"], 6),
-            # The bytes the part's format must move (DIA and block formats
-            # read no per-nonzero column index, so the algorithmic count
-            # above can exceed what they move and frac can pass 1).
-            "format_bytes": dom["format_bytes"],
-            "format_frac": round(dom["format_bytes"] / (dom["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4)
-            if dom["ms"] > 0 else 0.0,
-        },
-        "parts": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in p.items()} for p in parts],
-        "graph_ms": round(graph_ms, 6),
-        "cpu_baseline": cpu,
     }
+    for k, v in results[0].items():
+        if k not in out:
+            out[k] = v
+    if len(results) > 1:  # the power-law matrix's numbers, measured the same way
+        extra = {k: v for k, v in results[1].items() if k != "metric"}
+        out["powerlaw"] = extra
     print(json.dumps(out))
     return 0
 

@@ -51,8 +51,9 @@ BLOCK_SHAPES = [(1, 4), (2, 4), (4, 4), (8, 4), (1, 8), (2, 8), (4, 8), (8, 8)]
 
 
 class ClspmvError(RuntimeError):
-    def __init__(self, code: int, what: str):
-        msg = lib.clspmv_last_error().decode(errors="replace")
+    def __init__(self, code: int, what: str, msg: Optional[str] = None):
+        if msg is None:  # the library's message for its own failures
+            msg = lib.clspmv_last_error().decode(errors="replace")
         super().__init__(f"{what} failed with status {code}: {msg}")
         self.code = code
 
@@ -160,7 +161,9 @@ class DriverReport(C.Structure):
     _fields_ = [("format", C.c_int), ("variants", C.c_int), ("best_method", C.c_int),
                 ("best_bheight", C.c_int), ("best_bwidth", C.c_int), ("best_ms", C.c_double),
                 ("best_gflops", C.c_double), ("ms", C.c_double * 64), ("method", C.c_int * 64),
-                ("bheight", C.c_int * 64), ("bwidth", C.c_int * 64), ("max_rel_diff", C.c_double)]
+                ("bheight", C.c_int * 64), ("bwidth", C.c_int * 64), ("max_rel_diff", C.c_double),
+                ("check_distance", C.c_double), ("check_max_diff", C.c_double), ("check_max_ratio", C.c_double),
+                ("max_scaled_error", C.c_double), ("variants_over_tol", C.c_int)]
 
 
 class PartTiming(C.Structure):
@@ -288,6 +291,7 @@ if hasattr(lib, "clspmv_init_mat_kernels"):
     _sig("clspmv_init_mat_kernels", C.c_int, P(CocktailMatrix), PF, PF, P(VP), P(VP), C.c_int)
     _sig("clspmv_init_mat_kernels_on", C.c_int, C.c_int, P(CocktailMatrix), PF, PF, P(VP), P(VP))
     _sig("clspmv_do_spmv", C.c_int, VP, C.c_int)
+    _sig("clspmv_cocktail_time_cold", C.c_int, VP, C.c_int, C.c_size_t, P(C.c_double))
     _sig("clspmv_do_spmv_async", C.c_int, VP, C.c_int)
     for _n in ["vector_from_cpu", "result_from_cpu"]:
         _sig(f"clspmv_cpy_{_n}", C.c_int, VP, VP, PF, C.c_int)
@@ -655,9 +659,17 @@ class DeviceBuffer:
             pass
 
 
+def _nbytes(buf) -> int:
+    """Size in bytes of a DeviceBuffer or DeviceVector."""
+    return buf.nbytes if hasattr(buf, "nbytes") else 4 * buf.n
+
+
 def spmm_pack_x(ctx: Context, k: int, cols: int, X: DeviceBuffer, ldx: int) -> DeviceBuffer:
     """Column-major 16-bit X (k columns, leading dimension ldx) into the
     4-row tiles of clspmv_spmm_tiled, on the GPU (clspmv_spmm_pack_x)."""
+    need = 2 * (ldx * (k - 1) + cols) if k > 0 else 0
+    if _nbytes(X) < need:
+        raise ClspmvError(ERR_INVALID, "spmm_pack_x", f"X holds {_nbytes(X)} bytes, k={k} ldx={ldx} need {need}")
     X4 = DeviceBuffer(ctx, np.zeros(max(1, (cols + 3) // 4 * k * 4), np.uint16))
     check(lib.clspmv_spmm_pack_x(ctx.h, k, cols, X.p, ldx, X4.p), "spmm_pack_x")
     return X4
@@ -665,12 +677,20 @@ def spmm_pack_x(ctx: Context, k: int, cols: int, X: DeviceBuffer, ldx: int) -> D
 
 class ServerConfig(C.Structure):
     _fields_ = [("device_ordinal", C.c_int), ("streams", C.c_int), ("max_batch", C.c_int),
-                ("hbm_budget", C.c_size_t), ("use_graphs", C.c_int)]
+                ("hbm_budget", C.c_size_t), ("use_graphs", C.c_int), ("host_staging", C.c_size_t)]
 
 
 class Request(C.Structure):
-    _fields_ = [("matrix", C.c_int), ("k", C.c_int), ("seed", C.c_ulonglong), ("host_y", PF),
-                ("arrival", C.c_double)]
+    _fields_ = [("matrix", C.c_int), ("k", C.c_int), ("seed", C.c_ulonglong), ("x", PF),
+                ("x_on_device", C.c_int), ("host_y", PF), ("device_y", VP), ("arrival", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_longlong), ("passes", C.c_longlong), ("columns", C.c_longlong),
+                ("evictions", C.c_longlong), ("pool_bytes", C.c_size_t), ("pool_used", C.c_size_t)]
+
+
+_sig("clspmv_server_get_stats", C.c_int, VP, P(ServerStats))
 
 
 def request_x(seed: int, j: int, n: int) -> np.ndarray:
@@ -680,12 +700,26 @@ def request_x(seed: int, j: int, n: int) -> np.ndarray:
     return out
 
 
+def _request(matrix, k, seed, arrival=0.0, host_y=None, x=None, device_y=None) -> Request:
+    """A clspmv_request; x is a host array (cols * k floats, column-major) or
+    a DeviceVector / DeviceBuffer on the server's GPU."""
+    if x is None:
+        xp, xdev = None, 0
+    elif isinstance(x, np.ndarray):
+        assert x.dtype == np.float32 and x.flags.c_contiguous
+        xp, xdev = _fp(x), 0
+    else:
+        xp, xdev = C.cast(x.p, PF), 1
+    return Request(matrix, k, seed, xp, xdev, _fp(host_y) if host_y is not None else None,
+                   device_y.p if device_y is not None else None, arrival)
+
+
 class Server:
     """The request-serving loop (include/clspmv_serve.h)."""
 
     def __init__(self, device: int = 0, streams: int = 4, max_batch: int = 64,
-                 hbm_budget: int = 8 << 30, use_graphs: bool = True):
-        cfg = ServerConfig(device, streams, max_batch, hbm_budget, int(use_graphs))
+                 hbm_budget: int = 8 << 30, use_graphs: bool = True, host_staging: int = 0):
+        cfg = ServerConfig(device, streams, max_batch, hbm_budget, int(use_graphs), host_staging)
         h = VP()
         check(lib.clspmv_server_create(C.byref(cfg), C.byref(h)), "server_create")
         self.h = h
@@ -697,8 +731,9 @@ class Server:
         self._keep.append(dmat)
         return i.value
 
-    def submit(self, matrix: int, k: int, seed: int, host_y: Optional[np.ndarray] = None) -> int:
-        r = Request(matrix, k, seed, _fp(host_y) if host_y is not None else None, 0.0)
+    def submit(self, matrix: int, k: int, seed: int, host_y: Optional[np.ndarray] = None, x=None,
+               device_y=None) -> int:
+        r = _request(matrix, k, seed, 0.0, host_y, x, device_y)
         t = C.c_longlong()
         check(lib.clspmv_server_submit(self.h, C.byref(r), C.byref(t)), "server_submit")
         return t.value
@@ -708,12 +743,18 @@ class Server:
         check(lib.clspmv_server_wait(self.h, C.c_longlong(ticket), C.byref(lat)), "server_wait")
         return lat.value
 
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib.clspmv_server_get_stats(self.h, C.byref(st)), "server_get_stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
     def replay(self, reqs, mode: int = 0):
-        """reqs: list of (matrix, k, seed, arrival_s, host_y or None).
+        """reqs: list of (matrix, k, seed, arrival_s, host_y or None[, x[, device_y]]).
         Returns (latencies, wall seconds)."""
         arr = (Request * len(reqs))()
-        for i, (m, k, seed, t, y) in enumerate(reqs):
-            arr[i] = Request(m, k, seed, _fp(y) if y is not None else None, t)
+        for i, r in enumerate(reqs):
+            m, k, seed, t, y = r[:5]
+            arr[i] = _request(m, k, seed, t, y, r[5] if len(r) > 5 else None, r[6] if len(r) > 6 else None)
         lat = np.zeros(len(reqs), np.float64)
         wall = C.c_double()
         check(lib.clspmv_server_replay(self.h, len(reqs), arr, mode, lat.ctypes.data_as(P(C.c_double)),
@@ -743,12 +784,28 @@ class BlockSpmm:
         self.info = DmatInfo()
         check(lib.clspmv_dmat_get_info(h, C.byref(self.info)), "dmat_info")
 
+    def _check_y(self, k: int, Y, ldy: int):
+        need = 4 * (ldy * (k - 1) + self.info.rows) if k > 0 else 0
+        if _nbytes(Y) < need:
+            raise ClspmvError(ERR_INVALID, "spmm", f"Y holds {_nbytes(Y)} bytes, k={k} ldy={ldy} need {need}")
+
     def spmm(self, k: int, X, ldx: int, Y, ldy: int):
+        # The C entry point takes raw pointers and cannot see buffer sizes:
+        # a short X or Y is refused here instead of read or written past
+        # its end on the GPU.
+        need = 2 * (ldx * (k - 1) + self.info.cols) if k > 0 else 0
+        if _nbytes(X) < need:
+            raise ClspmvError(ERR_INVALID, "spmm", f"X holds {_nbytes(X)} bytes, k={k} ldx={ldx} need {need}")
+        self._check_y(k, Y, ldy)
         check(lib.clspmv_spmm(self.ctx.h, self.h, k, X.p, ldx, Y.p, ldy), "spmm")
 
     def spmm_tiled(self, k: int, X4, Y, ldy: int):
         """Y += A X with X in 4-row tiles (clspmv_spmm_tiled; spmm_pack_x
         builds them from a column-major X)."""
+        need = 8 * ((self.info.cols + 3) // 4) * k
+        if _nbytes(X4) < need:
+            raise ClspmvError(ERR_INVALID, "spmm_tiled", f"X4 holds {_nbytes(X4)} bytes, k={k} need {need}")
+        self._check_y(k, Y, ldy)
         check(lib.clspmv_spmm_tiled(self.ctx.h, self.h, k, X4.p, Y.p, ldy), "spmm_tiled")
 
     def error_report(self, ref: "DeviceMatrix", seed: int = 1, tol: float = 1e-5) -> dict:
@@ -894,6 +951,18 @@ def autotune(out_dir: str, device: int = 0, level: int = 0, verbose: bool = Fals
     return out_dir
 
 
+def evaluate(ck: Cocktail, x: np.ndarray, res: np.ndarray, coores: Optional[np.ndarray] = None) -> dict:
+    """clspmv_evaluate (evaluate, src/spmv_cocktail.cpp:1431): one SpMV
+    res += A x (res updated in place), the reference's comparison with
+    coores, then 1000 timed passes."""
+    assert res.dtype == np.float32 and res.flags.c_contiguous
+    rep = EvalReport()
+    xx = np.ascontiguousarray(x, np.float32)
+    check(lib.clspmv_evaluate(C.byref(ck.c), _fp(xx), _fp(res),
+                              _fp(coores) if coores is not None else None, C.byref(rep)), "evaluate")
+    return {f: getattr(rep, f) for f, _ in EvalReport._fields_}
+
+
 # Tables measured on an MI355X by autotune(), shipped with the package.
 TABLES_DIR = os.path.join(_HERE, "tables_mi355x")
 
@@ -955,6 +1024,14 @@ class CocktailRuntime:
         check(lib.clspmv_cocktail_profile(self.k, ntimes, parts, 16, C.byref(n), C.byref(gms)), "profile")
         return [dict(format=FORMAT_NAMES[parts[i].format], method=parts[i].method, nnz=parts[i].nnz,
                      stored=parts[i].stored, ms=parts[i].ms) for i in range(n.value)], gms.value
+
+    def time_cold(self, ntimes: int = 20, scrub_bytes: int = 1 << 30) -> float:
+        """ms per pass with the caches scrubbed before each pass
+        (clspmv_cocktail_time_cold)."""
+        ms = C.c_double()
+        check(lib.clspmv_cocktail_time_cold(self.k, ntimes, C.c_size_t(scrub_bytes), C.byref(ms)),
+              "cocktail_time_cold")
+        return ms.value
 
     def tune(self, ntimes: int = 20) -> bool:
         """Time every allowed method (and lane count) per part, keep the

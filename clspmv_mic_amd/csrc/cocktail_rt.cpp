@@ -17,6 +17,7 @@
 // ragged end; its update still reaches y before the second pass's.
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 
@@ -427,6 +428,50 @@ int clspmv_cocktail_profile(clspmv_cocktail_kernels* k, int ntimes, clspmv_part_
     return CLSPMV_OK;
 }
 
+int clspmv_cocktail_time_cold(clspmv_cocktail_kernels* k, int ntimes, size_t scrub_bytes, double* ms_per_pass) {
+    if (!k || !k->gpu || !k->exec || ntimes <= 0 || !ms_per_pass) return fail(CLSPMV_ERR_INVALID, "bad argument");
+    int st = use_ctx_device(k->ctx);
+    if (st) return st;
+    clspmv_cocktail_gpu* g = k->gpu;
+    hipStream_t s = k->ctx->stream;
+    const size_t half = std::max<size_t>(scrub_bytes / 2 / 256 * 256, 256);
+    float* saved = nullptr;
+    char* scrub = nullptr;
+    CLSPMV_HIP(hipMalloc(&saved, sizeof(float) * std::max(g->height, 1)));
+    hipError_t e = hipMalloc(&scrub, 2 * half);
+    if (e != hipSuccess) {
+        (void)hipFree(saved);
+        return fail(CLSPMV_ERR_ALLOC, "scrub buffer of %zu bytes: %s", 2 * half, hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(scrub, 0, 2 * half, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(saved, g->res, sizeof(float) * g->height, hipMemcpyDeviceToDevice, s);
+    double total = 0.0;
+    for (int r = 0; r < ntimes && e == hipSuccess && !st; ++r) {
+        e = hipMemcpyAsync(scrub + half, scrub, half, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(k->ctx->ev_start, s);
+        if (e != hipSuccess) break;
+        if (k->use_graph)
+            e = hipGraphLaunch(k->exec, s);
+        else
+            st = launch_all(s, k);
+        if (e == hipSuccess && !st) e = hipEventRecord(k->ctx->ev_stop, s);
+        if (e == hipSuccess && !st) e = hipEventSynchronize(k->ctx->ev_stop);
+        float ms = 0.0f;
+        if (e == hipSuccess && !st) e = hipEventElapsedTime(&ms, k->ctx->ev_start, k->ctx->ev_stop);
+        total += ms;
+    }
+    hipError_t e2 = hipMemcpyAsync(g->res, saved, sizeof(float) * g->height, hipMemcpyDeviceToDevice, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(saved);
+    (void)hipFree(scrub);
+    if (st) return st;
+    if (e != hipSuccess || e2 != hipSuccess)
+        return fail(CLSPMV_ERR_HIP, "cold timing failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    *ms_per_pass = total / ntimes;
+    return CLSPMV_OK;
+}
+
 namespace {
 
 // Average ms of `ntimes` runs of `fn` on the kernels' stream (one untimed
@@ -454,8 +499,18 @@ std::vector<int> lane_candidates(int lanes) {
     return c;
 }
 
+// Trial number (counted from 1 over one tuning call) at which tune_parts
+// fails on purpose, from CLSPMV_TUNE_FAIL_TRIAL; 0 = never.  A host-side
+// error for tests of the recovery path: no kernel is involved.
+int injected_fail_trial() {
+    const char* v = std::getenv("CLSPMV_TUNE_FAIL_TRIAL");
+    return v ? std::atoi(v) : 0;
+}
+
 int tune_parts(clspmv_cocktail_kernels* k, int ntimes) {
     clspmv_cocktail_gpu* g = k->gpu;
+    const int fail_at = injected_fail_trial();
+    int trials = 0;
     for (size_t i = 0; i < g->parts.size(); ++i) {
         clspmv_dmat* A = g->parts[i];
         const int fmt = A->info.format;
@@ -464,8 +519,11 @@ int tune_parts(clspmv_cocktail_kernels* k, int ntimes) {
         bool best_panel = A->use_panel;
         const bool lane_fmt = fmt == CLSPMV_FMT_CSR || fmt == CLSPMV_FMT_BCSR;
         const std::vector<int> lanes = lane_fmt ? lane_candidates(A->lanes) : std::vector<int>{A->lanes};
-        const int upload_lanes = A->lanes;
+        // The part's state before its trials, restored if one fails.
+        const int upload_lanes = A->lanes, upload_method = g->methods[i];
+        const bool upload_panel = A->use_panel;
         auto trial = [&](int m, int l, bool panel) -> int {
+            if (++trials == fail_at) return fail(CLSPMV_ERR_HIP, "tune: injected failure at trial %d", trials);
             A->lanes = l;
             A->use_panel = panel;
             g->methods[i] = m;
@@ -500,8 +558,8 @@ int tune_parts(clspmv_cocktail_kernels* k, int ntimes) {
         }
         if (st) {
             A->lanes = upload_lanes;
-            A->use_panel = A->pw && !A->plain;
-            g->methods[i] = best_m;
+            A->use_panel = upload_panel;
+            g->methods[i] = upload_method;
             return st;
         }
         g->methods[i] = best_m;
@@ -573,7 +631,15 @@ int clspmv_cocktail_tune(clspmv_cocktail_kernels* k, int ntimes, int* graph) {
     e = hipMemcpyAsync(g->res, saved, ybytes, hipMemcpyDeviceToDevice, s);
     hipError_t e2 = hipStreamSynchronize(s);
     (void)hipFree(saved);
-    if (st) return st;
+    if (st) {
+        // Parts tuned before the failure may have dropped an encoding that
+        // the graph captured earlier still points at: capture the pass again
+        // from the parts' current state.  If that fails too, capture() has
+        // left no graph, and do_spmv refuses to run.
+        const int err = st;
+        (void)capture(k);
+        return err;
+    }
     if (e != hipSuccess || e2 != hipSuccess)
         return fail(CLSPMV_ERR_HIP, "tune: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     k->use_graph = best_graph;
@@ -627,22 +693,11 @@ int clspmv_evaluate(const clspmv_cocktail* cpumat, const float* vec, float* res,
     if (!st) st = clspmv_cpy_result_to_cpu(g, k, y.data(), h);
     clspmv_eval_report rep{};
     if (!st && coores) {
-        // The reference's report (src/util.cpp:93-131): L2 distance, largest
-        // difference of magnitudes and its ratio to the larger magnitude.
-        double dist = 0, maxdiff = 0, maxratio = 0;
-        for (int i = 0; i < h; ++i) {
-            // Differences in float, then widened (src/util.cpp:79,108).
-            double t = (double)(float)(coores[i] - y[i]);
-            dist += t * t;
-            float a = std::fabs(coores[i]), b = std::fabs(y[i]);
-            double d = std::fabs((double)(float)(a - b));
-            float mx = a > b ? a : b;
-            if (d > maxdiff) maxdiff = d;
-            if (mx > 0 && d / mx > maxratio) maxratio = d / mx;
-        }
-        rep.l2_distance = std::sqrt(dist);
-        rep.max_diff = maxdiff;
-        rep.max_ratio = maxratio;
+        // The reference's report (src/util.cpp:93-131).
+        const VecCompare c = two_vec_compare(coores, y.data(), h);
+        rep.l2_distance = c.distance;
+        rep.max_diff = c.max_diff;
+        rep.max_ratio = c.max_ratio;
         std::printf("\nCorrectness Check: Distance %e max diff %e max diff ratio %e vec size %d\n",
                     rep.l2_distance, rep.max_diff, rep.max_ratio, h);
     }

@@ -63,4 +63,14 @@ bool coo_sorted(const clspmv_coo_matrix& m);
 // Sort entries by (row, col), keeping the input order of equal keys.
 void coo_sort(clspmv_coo_matrix& m);
 
+// The reference's correctness report of a result against the serial one
+// (two_vec_compare, src/util.cpp:93-131, with distance :74): L2 norm of the
+// differences (each taken in float, then widened), the largest difference
+// of magnitudes, and the largest such difference relative to the larger
+// magnitude.
+struct VecCompare {
+    double distance = 0.0, max_diff = 0.0, max_ratio = 0.0;
+};
+VecCompare two_vec_compare(const float* coovec, const float* newvec, int n);
+
 }  // namespace clspmv

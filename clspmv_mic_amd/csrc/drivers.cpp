@@ -6,9 +6,11 @@
 // shapes), uploads it, and for every kernel variant: zeroes y, runs one SpMV
 // with x = 1, keeps the result, then times `ntimes` launches after three
 // warm-up launches (src/spmv_bdia.cpp:111-137).  It prints one line per
-// variant and the best one, and fills a report.  Variants are compared with
-// each other (max relative difference); comparing with an independent CPU
-// result is left to the caller.
+// variant and the best one, and fills a report.  Like the reference, every
+// variant's first result is checked against the serial COO SpMV of the same
+// matrix on the host (spmv_only + two_vec_compare, src/spmv_bdia.cpp:118,
+// :422-423), and also scored against an FP64 host product by the row-scaled
+// tolerance |y - y64| <= 1e-5 (|A||x|); variants over it are counted.
 
 #include <algorithm>
 #include <cmath>
@@ -26,12 +28,16 @@ struct Variant {
     int bh, bw;
 };
 
+constexpr double kDriverTol = 1e-5;
+
 struct DriverRun {
     clspmv_ctx* ctx = nullptr;
     float* x = nullptr;
     float* y = nullptr;
     int rows = 0, cols = 0;
     std::vector<float> first;
+    std::vector<float> coores;         // serial COO SpMV, x = 1 (spmv_only, src/util.cpp:85)
+    std::vector<double> y64, bound;    // FP64 product and sum |a x| per row
     ~DriverRun() {
         if (ctx) {
             if (x) (void)hipFree(x);
@@ -41,11 +47,31 @@ struct DriverRun {
     }
 };
 
-int driver_setup(DriverRun& R, int device, const clspmv_coo_matrix* mat) {
+// The host references of a driver run for x = 1: the reference's serial
+// check (entries in storage order, FP32) and an FP64 product with its
+// per-row bound.  The matrix is sorted first, as every conversion of the
+// reference sorts it in place (include/matrix_storage.h:1271).
+void host_references(DriverRun& R, clspmv_coo_matrix* mat) {
+    if (!coo_sorted(*mat)) coo_sort(*mat);
+    const int nnz = mat->matinfo.nnz;
+    R.coores.assign(std::max(R.rows, 1), 0.0f);
+    R.y64.assign(std::max(R.rows, 1), 0.0);
+    R.bound.assign(std::max(R.rows, 1), 0.0);
+    for (int i = 0; i < nnz; ++i) {
+        const int r = mat->coo_row_id[i];
+        const float a = mat->coo_data[i];
+        R.coores[r] += a * 1.0f;
+        R.y64[r] += (double)a;
+        R.bound[r] += std::fabs((double)a);
+    }
+}
+
+int driver_setup(DriverRun& R, int device, clspmv_coo_matrix* mat) {
     int st = clspmv_ctx_create(device, &R.ctx);
     if (st) return st;
     R.rows = mat->matinfo.height;
     R.cols = mat->matinfo.width;
+    host_references(R, mat);
     CLSPMV_HIP(hipMalloc(&R.x, sizeof(float) * std::max(R.cols, 1)));
     CLSPMV_HIP(hipMalloc(&R.y, sizeof(float) * std::max(R.rows, 1)));
     std::vector<float> ones(std::max(R.cols, 1), 1.0f);
@@ -70,6 +96,22 @@ int run_variant(DriverRun& R, const Variant& v, int ntimes, const char* name,
             if (m > 0) rep->max_rel_diff = std::max(rep->max_rel_diff, std::fabs((double)y[i] - R.first[i]) / m);
         }
     }
+    // The reference's check against the serial result, and the tolerance.
+    const VecCompare c = two_vec_compare(R.coores.data(), y.data(), R.rows);
+    std::printf("\nCorrectness Check: Distance %e max diff %e max diff ratio %e vec size %d\n", c.distance,
+                c.max_diff, c.max_ratio, R.rows);
+    rep->check_distance = std::max(rep->check_distance, c.distance);
+    rep->check_max_diff = std::max(rep->check_max_diff, c.max_diff);
+    rep->check_max_ratio = std::max(rep->check_max_ratio, c.max_ratio);
+    double worst = 0.0;
+    for (int i = 0; i < R.rows; ++i) {
+        const double d = std::fabs((double)y[i] - R.y64[i]);
+        double e = R.bound[i] > 0 ? d / R.bound[i] : (d > 0 ? INFINITY : 0.0);
+        if (e != e) e = INFINITY;
+        worst = std::max(worst, e);
+    }
+    rep->max_scaled_error = std::max(rep->max_scaled_error, worst);
+    if (worst > kDriverTol) rep->variants_over_tol++;
     double ms = 0.0;
     st = clspmv_time_spmv(R.ctx, v.A, v.method, R.x, R.y, 3, std::max(ntimes, 1), &ms);
     if (st) return st;
@@ -129,9 +171,10 @@ int drive(const char* name, int format, clspmv_coo_matrix* mat, int ntimes,
         clspmv_dmat_free(A);
     }
     if (rep->variants == 0) return fail(CLSPMV_ERR_UNSUPPORTED, "%s: no variant ran", name);
-    std::printf("%s best: method %d block %dx%d %.6f ms %.3f GFLOPS (max variant diff %.3e)\n",
+    std::printf("%s best: method %d block %dx%d %.6f ms %.3f GFLOPS (max variant diff %.3e, "
+                "max scaled error %.3e, %d variants over %.0e)\n",
                 name, rep->best_method, rep->best_bheight, rep->best_bwidth, rep->best_ms,
-                rep->best_gflops, rep->max_rel_diff);
+                rep->best_gflops, rep->max_rel_diff, rep->max_scaled_error, rep->variants_over_tol, kDriverTol);
     return CLSPMV_OK;
 }
 

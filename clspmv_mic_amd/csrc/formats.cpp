@@ -21,6 +21,7 @@
 #include <cassert>
 #include <cerrno>
 #include <climits>
+#include <cmath>
 #include <cstdint>
 #include <numeric>
 
@@ -47,6 +48,22 @@ void row_pointer(const int* rows, int nnz, int height, std::vector<int>& rp) {
     rp.assign((size_t)height + 1, 0);
     for (int i = 0; i < nnz; ++i) rp[rows[i] + 1]++;
     for (int r = 0; r < height; ++r) rp[r + 1] += rp[r];
+}
+
+VecCompare two_vec_compare(const float* coovec, const float* newvec, int n) {
+    VecCompare c;
+    double dist = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double t = (double)(float)(coovec[i] - newvec[i]);
+        dist += t * t;
+        const float a = std::fabs(coovec[i]), b = std::fabs(newvec[i]);
+        const double d = std::fabs((double)(float)(a - b));
+        const float mx = a > b ? a : b;
+        if (d > c.max_diff) c.max_diff = d;
+        if (mx > 0 && d / mx > c.max_ratio) c.max_ratio = d / mx;
+    }
+    c.distance = std::sqrt(dist);
+    return c;
 }
 
 bool coo_sorted(const clspmv_coo_matrix& m) {

@@ -243,7 +243,17 @@ hipError_t csr_multi_kb(hipStream_t s, int nblocks, const int* blocks, const int
 
 template <class V>
 hipError_t csr_multi_v(hipStream_t s, int nblocks, const int* blocks, const int* rp, const int* col,
-                       const void* val, int k, const float* X, float* Y, int ldy) {
+                       const void* val, int k, const float* X, float* Y, int ldy, int kb) {
+    if (kb > 0) {  // a fixed width: the summation order of each right-hand side does not depend on k
+        switch (kb) {
+            case 1: return csr_multi_kb<1, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+            case 2: return csr_multi_kb<2, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+            case 4: return csr_multi_kb<4, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+            case 8: return csr_multi_kb<8, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+            case 16: return csr_multi_kb<16, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+            default: return hipErrorInvalidValue;
+        }
+    }
     if (k <= 1) return csr_multi_kb<1, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
     if (k <= 2) return csr_multi_kb<2, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
     if (k <= 4) return csr_multi_kb<4, V>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
@@ -268,13 +278,13 @@ hipError_t dia_multi(hipStream_t s, int rows, int cols, int len, int ndiag, cons
 }
 
 hipError_t csr_multi(hipStream_t s, int nblocks, const int* blocks, const int* rp, const int* col,
-                     int dtype, const void* val, int k, const float* X, float* Y, int ldy) {
+                     int dtype, const void* val, int k, const float* X, float* Y, int ldy, int kb) {
     if (nblocks <= 0 || k <= 0) return hipSuccess;
-    if (k > kMultiMax) return hipErrorInvalidValue;
+    if (k > kMultiMax || (kb > 0 && k > kb)) return hipErrorInvalidValue;
     switch (dtype) {
-        case 0: return csr_multi_v<ValF32>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
-        case 1: return csr_multi_v<ValBF16>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
-        case 2: return csr_multi_v<ValF16>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy);
+        case 0: return csr_multi_v<ValF32>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy, kb);
+        case 1: return csr_multi_v<ValBF16>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy, kb);
+        case 2: return csr_multi_v<ValF16>(s, nblocks, blocks, rp, col, val, k, X, Y, ldy, kb);
         default: return hipErrorInvalidValue;
     }
 }

@@ -130,9 +130,12 @@ hipError_t sbell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
 constexpr int kMultiMax = 16;
 hipError_t dia_multi(hipStream_t s, int rows, int cols, int len, int ndiag, const int* offsets,
                      int dtype, const void* data, int k, const float* X, float* Y, int ldy);
-// CSR over the adaptive row blocks (csr_adaptive's `blocks`).
+// CSR over the adaptive row blocks (csr_adaptive's `blocks`).  kb > 0 runs
+// the kernel built for kb right-hand sides (k <= kb; 1, 2, 4, 8 or 16)
+// whatever k is, so that each right-hand side's summation order stays that
+// of kb; kb = 0 picks the smallest width that holds k.
 hipError_t csr_multi(hipStream_t s, int nblocks, const int* blocks, const int* rp, const int* col,
-                     int dtype, const void* val, int k, const float* X, float* Y, int ldy);
+                     int dtype, const void* val, int k, const float* X, float* Y, int ldy, int kb = 0);
 
 // ---- mixed-precision 4x4-block SpMM (MFMA) -------------------------------------
 // Y[:, 0:k] += A X[:, 0:k]; A in slices of kSpmmSlice block rows: sptr[s] is

@@ -234,7 +234,7 @@ bool has_multi(const clspmv_dmat* A) {
     return A->info.format == CLSPMV_FMT_DIA || (A->info.format == CLSPMV_FMT_CSR && A->plain);
 }
 
-int multi_launch(hipStream_t s, const clspmv_dmat* A, int k, const float* X, float* Y, int ldy) {
+int multi_launch(hipStream_t s, const clspmv_dmat* A, int k, const float* X, float* Y, int ldy, int kb) {
     const clspmv_dmat_info& I = A->info;
     hipError_t e;
     if (I.format == CLSPMV_FMT_DIA) {
@@ -242,7 +242,7 @@ int multi_launch(hipStream_t s, const clspmv_dmat* A, int k, const float* X, flo
                               A->dtype ? (const void*)A->lpval : (const void*)A->val, k, X, Y, ldy);
     } else if (I.format == CLSPMV_FMT_CSR && A->plain) {
         e = launch::csr_multi(s, A->nblocks, A->blocks, A->rp, A->col, A->dtype,
-                              A->dtype ? (const void*)A->lpval : (const void*)A->val, k, X, Y, ldy);
+                              A->dtype ? (const void*)A->lpval : (const void*)A->val, k, X, Y, ldy, kb);
     } else {
         return fail(CLSPMV_ERR_UNSUPPORTED, "multi-vector SpMV exists for DIA and CSR (own arrays) only");
     }
@@ -936,7 +936,7 @@ int clspmv_dmat_set_panel(clspmv_dmat* m, int use_panel) {
     if (use_panel && !m->pw) return fail(CLSPMV_ERR_UNSUPPORTED, "matrix has no panel encoding");
     if (!use_panel && !m->plain) return fail(CLSPMV_ERR_UNSUPPORTED, "matrix has only the panel encoding");
     m->use_panel = use_panel != 0;
-    m->multi_pick.clear();
+    m->serve_multi = -1;
     return CLSPMV_OK;
 }
 

@@ -61,10 +61,12 @@ struct clspmv_dmat {
     int* pblk_long = nullptr;  // per block: first of its long rows in plong (+ end)
     int* plong = nullptr;      // long rows: {row in block, first slot, end slot, 0}
     bool use_panel = false;
-    // Serving: per k, 1 when one multi-vector pass measured faster than k
-    // SpMVs, 0 when not, -1 not measured yet.  Kept with the matrix so that
-    // every server makes the same choice; cleared when the encoding changes.
-    mutable std::vector<signed char> multi_pick;
+    // Serving: 1 when batched passes run the multi-vector kernels (they
+    // measured faster than separate SpMVs when the matrix was first
+    // registered), 0 when not, -1 not measured yet.  Kept with the matrix so
+    // that every server runs the same kernels; cleared when the encoding
+    // changes.  Written only while a server holds the matrix's claim.
+    mutable int serve_multi = -1;
     bool plain = true;
     // Values stored in 16 bits (dtype CLSPMV_DT_BF16 / CLSPMV_DT_F16): DIA
     // data, and BSPMM blocks (16 values per 4x4 block, row-major; rp / col
@@ -158,9 +160,12 @@ int dmat_launch_rest(hipStream_t stream, const clspmv_dmat* A, int method,
                      const float* x, float* y);
 
 // Multi-vector SpMV (DIA, CSR with its own arrays): Y[:, q] += A X[:, q],
-// X row-major, Y column-major; no argument checks.
+// X row-major, Y column-major; no argument checks.  kb > 0: CSR runs the
+// kernel built for kb right-hand sides whatever k is (launch::csr_multi);
+// each right-hand side's result then does not depend on k.  DIA's never
+// does.
 bool has_multi(const clspmv_dmat* A);
-int multi_launch(hipStream_t s, const clspmv_dmat* A, int k, const float* X, float* Y, int ldy);
+int multi_launch(hipStream_t s, const clspmv_dmat* A, int k, const float* X, float* Y, int ldy, int kb = 0);
 
 // Method chosen for CLSPMV_METHOD_AUTO.
 int auto_method(const clspmv_dmat* A);

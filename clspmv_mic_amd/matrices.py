@@ -51,36 +51,49 @@ def laplacian_2d(n: int = 256) -> Coo:
 
 
 def stencil_27pt(n: int = 128, seed: int = 3, dyadic: bool = False) -> Coo:
-    """27-point stencil; rows in natural (x fastest) order."""
+    """27-point stencil; rows in natural (x fastest) order.
+
+    Values are drawn neighbour by neighbour in (dz, dy, dx) order, each
+    draw covering the rows that have that neighbour; since the column
+    offsets dz*n^2 + dy*n + dx increase in that same order, filling a
+    rows x 27 table and dropping the missing neighbours yields the entries
+    already sorted by (row, column)."""
     m = n ** 3
     rng = np.random.default_rng(seed)
     r = np.arange(m, dtype=np.int64)
     x, y, z = r % n, (r // n) % n, r // (n * n)
-    rows, cols, vals = [], [], []
+    val = np.zeros((m, 27), np.float32)
+    ok_all = np.zeros((m, 27), bool)
+    offs = np.zeros(27, np.int64)
     absum = np.zeros(m, np.float64)
+    j = 0
     for dz in (-1, 0, 1):
         for dy in (-1, 0, 1):
             for dx in (-1, 0, 1):
+                offs[j] = dx + dy * n + dz * n * n
                 if dx == dy == dz == 0:
+                    j += 1
                     continue
                 ok = ((x + dx >= 0) & (x + dx < n) & (y + dy >= 0) & (y + dy < n) &
                       (z + dz >= 0) & (z + dz < n))
-                rr = r[ok]
+                cnt = int(ok.sum())
                 if dyadic:
-                    v = (rng.integers(-64, 65, size=rr.size) / 64.0).astype(np.float32)
+                    v = (rng.integers(-64, 65, size=cnt) / 64.0).astype(np.float32)
                 else:
-                    v = rng.standard_normal(rr.size).astype(np.float32)
-                absum[rr] += np.abs(v)
-                rows.append(rr.astype(np.int32))
-                cols.append((rr + dx + dy * n + dz * n * n).astype(np.int32))
-                vals.append(v)
+                    v = rng.standard_normal(cnt).astype(np.float32)
+                absum[ok] += np.abs(v)
+                val[ok, j] = v
+                ok_all[:, j] = ok
+                j += 1
     centre = (absum + 1.0).astype(np.float32)
     if dyadic:
         centre = np.ceil(absum + 1.0).astype(np.float32)
-    rows.append(r.astype(np.int32))
-    cols.append(r.astype(np.int32))
-    vals.append(centre)
-    return _finish(np.concatenate(rows), np.concatenate(cols), np.concatenate(vals), m, m)
+    val[:, 13] = centre
+    ok_all[:, 13] = True
+    del x, y, z, absum
+    rows = np.broadcast_to(r.astype(np.int32)[:, None], (m, 27))[ok_all]
+    cols = (r[:, None] + offs[None, :]).astype(np.int32)[ok_all]
+    return Coo(rows, cols, val[ok_all], m, m)
 
 
 def power_law(m: int = 1_000_000, seed: int = 2, s: float = 1.8, max_len: int = 5000,
@@ -110,19 +123,37 @@ BLOCK_OFFSETS = (0, 1, -1, 32, -32, 33, -33, 4096, -4096, 4097, -4097, 31)
 
 def block_stencil(nbr: int = 524_288, bh: int = 4, bw: int = 4, seed: int = 4,
                   offsets=BLOCK_OFFSETS, dyadic: bool = False) -> Coo:
-    """Block-row pattern with len(offsets) dense bh x bw blocks per block row."""
+    """Block-row pattern with len(offsets) dense bh x bw blocks per block row.
+
+    Values are drawn block by block (block row, then offset, then the
+    block's entries row-major); the entries are then laid out by (row,
+    column): each block row's blocks sorted by block column.  Where two
+    offsets meet the same block column (small nbr), the first block drawn
+    is kept."""
     rng = np.random.default_rng(seed)
     nbc = nbr  # square in blocks
-    br = np.repeat(np.arange(nbr, dtype=np.int64), len(offsets))
-    bc = (br + np.tile(np.array(offsets, dtype=np.int64), nbr)) % nbc
-    hh, ww = np.meshgrid(np.arange(bh), np.arange(bw), indexing="ij")
-    rows = (br[:, None] * bh + hh.ravel()[None, :]).ravel()
-    cols = (bc[:, None] * bw + ww.ravel()[None, :]).ravel()
+    nb = len(offsets)
+    bc = (np.arange(nbr, dtype=np.int64)[:, None] + np.array(offsets, dtype=np.int64)[None, :]) % nbc
     if dyadic:
-        vals = (rng.integers(-64, 65, size=rows.size) / 64.0).astype(np.float32)
+        vals = (rng.integers(-64, 65, size=nbr * nb * bh * bw) / 64.0).astype(np.float32)
     else:
-        vals = rng.standard_normal(rows.size).astype(np.float32)
-    return _finish(rows.astype(np.int32), cols.astype(np.int32), vals, nbr * bh, nbc * bw)
+        vals = rng.standard_normal(nbr * nb * bh * bw).astype(np.float32)
+    perm = np.argsort(bc, axis=1, kind="stable")
+    bcs = np.take_along_axis(bc, perm, axis=1)
+    if np.any(bcs[:, 1:] == bcs[:, :-1]):
+        # Colliding offsets: general path (sort + first occurrence wins).
+        br = np.repeat(np.arange(nbr, dtype=np.int64), nb)
+        hh, ww = np.meshgrid(np.arange(bh), np.arange(bw), indexing="ij")
+        rows = (br[:, None] * bh + hh.ravel()[None, :]).ravel()
+        cols = (bc.ravel()[:, None] * bw + ww.ravel()[None, :]).ravel()
+        return _finish(rows.astype(np.int32), cols.astype(np.int32), vals, nbr * bh, nbc * bw)
+    V = vals.reshape(nbr, nb, bh, bw)
+    V = np.take_along_axis(V, perm[:, :, None, None], axis=1)      # blocks by column
+    V = V.transpose(0, 2, 1, 3)                                       # (block row, h, block, w)
+    rows = np.repeat(np.arange(nbr * bh, dtype=np.int32), nb * bw)
+    cols = (bcs[:, None, :, None] * bw + np.arange(bw)[None, None, None, :])
+    cols = np.broadcast_to(cols, (nbr, bh, nb, bw)).astype(np.int32)
+    return Coo(rows, cols.ravel(), np.ascontiguousarray(V).ravel(), nbr * bh, nbc * bw)
 
 
 def random_sparse(height: int, width: int, density: float = 0.05, seed: int = 0,

@@ -154,6 +154,14 @@ int clspmv_cocktail_profile(clspmv_cocktail_kernels* kernels, int ntimes,
                             clspmv_part_timing* parts, int max_parts,
                             int* nparts, double* graph_ms);
 
+/* Cold-cache timing (no counterpart in the reference): ntimes passes in the
+ * kernels' launch mode, each after a device-to-device copy of scrub_bytes / 2
+ * bytes through a buffer of its own (reads and writes scrub_bytes, evicting
+ * the L2s and the memory-side cache), each pass alone between HIP events;
+ * *ms_per_pass is the average.  y is preserved.  Blocks. */
+int clspmv_cocktail_time_cold(clspmv_cocktail_kernels* kernels, int ntimes, size_t scrub_bytes,
+                              double* ms_per_pass);
+
 /* Measured launch tuning, the device-side counterpart of the reference's
  * runtime variant probe (init_mat_kernels with use_tex,
  * src/spmv_cocktail.cpp:1590): every method a part's shape allows (and,

@@ -270,6 +270,10 @@ int clspmv_gather_probe(clspmv_ctx* ctx, int nx, long long n, int ntimes,
  * the OpenCL device type, and the outcome is also returned in report.
  * y_out (optional, rows floats) receives A*x of the best variant for
  * x[i] = 1 (the all-ones vector of src/eval.cpp:119), y starting at zero.
+ * As in the reference, every variant is checked against the serial COO
+ * SpMV on the host and the comparison printed (src/spmv_bdia.cpp:118);
+ * the report carries it and an FP64 tolerance check.  The COO matrix is
+ * sorted in place when it is not (every reference conversion does so).
  * Block formats sweep the eight reference block shapes. */
 typedef struct {
     int    format;
@@ -283,6 +287,15 @@ typedef struct {
     int    bheight[64], bwidth[64];
     double max_rel_diff;          /* largest relative difference between
                                      any variant's y and the first's    */
+    /* The reference's check of each variant against the serial COO SpMV
+     * on the host (spmv_only + two_vec_compare, src/util.cpp:85-131):
+     * largest distance, max diff and max diff ratio over the variants. */
+    double check_distance, check_max_diff, check_max_ratio;
+    /* Largest row-scaled error |y - y64| / (sum_j |a_ij x_j|) of any
+     * variant against an FP64 host product, and the number of variants
+     * above 1e-5. */
+    double max_scaled_error;
+    int    variants_over_tol;
 } clspmv_driver_report;
 
 int clspmv_spmv_csr_scalar(const char* kernel_file, clspmv_coo_matrix* mat, int dim2Size, int ntimes, int device_ordinal, clspmv_driver_report* report, float* y_out);

@@ -258,7 +258,9 @@ int clspmv_coo2b4csr_nocap(clspmv_coo_matrix* src, clspmv_b4csr_matrix* dst, uns
 int clspmv_coo2b4ell_nocap(clspmv_coo_matrix* src, clspmv_b4ell_matrix* dst, unsigned int bwidth, unsigned int bheight, int alignment, int ellnum);
 int clspmv_coo2sbell_nocap(clspmv_coo_matrix* src, clspmv_sbell_matrix* dst, unsigned int bwidth, unsigned int bheight, unsigned int sliceheight);
 
-/* ---- X -> COO (dropping stored zeros, as the reference does) ---------- */
+/* ---- X -> COO: sorted by (row, col); csr2coo copies every stored entry,
+ * the others skip stored zeros (so the formats' padding goes), as the
+ * reference's loops do, and set nnz to the entries kept ---------------- */
 int clspmv_csr2coo(const clspmv_csr_matrix* src, clspmv_coo_matrix* dst);     /* :1240 */
 int clspmv_ell2coo(const clspmv_ell_matrix* src, clspmv_coo_matrix* dst);     /* :1310 */
 int clspmv_dia2coo(const clspmv_dia_matrix* src, clspmv_coo_matrix* dst);     /* :826  */

@@ -81,6 +81,13 @@ class Oracle:
     def csr_spmv(self, rp, col, val, x, y):
         self.lib.oracle_csr_spmv(C.c_int(len(rp) - 1), _i(rp), _i(col), _f(val), _f(x), _f(y))
 
+    def csr_spmv_threads(self, threads, rp, col, val, x, y):
+        self.lib.oracle_csr_spmv_threads(C.c_int(threads), C.c_int(len(rp) - 1), _i(rp), _i(col), _f(val),
+                                         _f(x), _f(y))
+
+    def max_threads(self) -> int:
+        return int(self.lib.oracle_max_threads())
+
     def ell_spmv(self, rows, hal, ellnum, col, val, x, y):
         self.lib.oracle_ell_spmv(rows, hal, ellnum, _i(col), _f(val), _f(x), _f(y))
 
@@ -148,6 +155,7 @@ class Reference:
         """ref_coo2<name>(coo, dst, *args); returns (status, struct)."""
         cs = self.cs
         T = {"csr": cs.CsrMatrix, "ell": cs.EllMatrix, "sell": cs.SellMatrix, "dia": cs.DiaMatrix,
+             "diaext": cs.DiaExtMatrix,
              "bdia": cs.BdiaMatrix, "bcsr": cs.BcsrMatrix, "b4csr": cs.B4csrMatrix,
              "bell": cs.BellMatrix, "b4ell": cs.B4ellMatrix, "sbell": cs.SbellMatrix}[name]
         dst = T()
@@ -158,6 +166,19 @@ class Reference:
         conv = [C.c_uint(a) if i < unsigned else C.c_int(a) for i, a in enumerate(args)]
         rc = f(C.byref(coo.c), C.byref(dst), *conv)
         return rc, dst
+
+    def to_coo(self, name: str, struct):
+        """ref_<name>2coo(struct, dst): the reference's inverse conversion;
+        returns the COO arrays (copied) as (row, col, val, height, width)."""
+        cs = self.cs
+        dst = cs.CooMatrix()
+        getattr(self.lib, f"ref_{name}2coo")(C.byref(struct), C.byref(dst))
+        n = dst.matinfo.nnz
+        row = np.ctypeslib.as_array(dst.coo_row_id, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+        col = np.ctypeslib.as_array(dst.coo_col_id, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+        val = np.ctypeslib.as_array(dst.coo_data, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+        self.lib.ref_free_coo(C.byref(dst))
+        return row, col, val, dst.matinfo.height, dst.matinfo.width
 
     def spmv(self, name: str, struct, x, y):
         """The reference's serial <name>_spmv (include/spmv_serial.h) on a struct."""

@@ -30,6 +30,7 @@ typedef csr_matrix<int, float> rcsr;
 typedef ell_matrix<int, float> rell;
 typedef sell_matrix<int, float> rsell;
 typedef dia_matrix<int, int, float> rdia;
+typedef dia_ext_matrix<int, int, float> rdiaext;
 typedef bdia_matrix<int, int, float> rbdia;
 typedef bcsr_matrix<int, float> rbcsr;
 typedef bell_matrix<int, float> rbell;
@@ -49,6 +50,8 @@ SAME(clspmv_sell_matrix, rsell, sell_slice_height); SAME(clspmv_sell_matrix, rse
 SAME(clspmv_sell_matrix, rsell, sell_data);
 SAME_SIZE(clspmv_dia_matrix, rdia);
 SAME(clspmv_dia_matrix, rdia, dia_length_aligned); SAME(clspmv_dia_matrix, rdia, dia_data);
+SAME_SIZE(clspmv_dia_ext_matrix, rdiaext);
+SAME(clspmv_dia_ext_matrix, rdiaext, dia_length_aligned); SAME(clspmv_dia_ext_matrix, rdiaext, dia_data);
 SAME_SIZE(clspmv_bdia_matrix, rbdia);
 SAME(clspmv_bdia_matrix, rbdia, bdia_bptr); SAME(clspmv_bdia_matrix, rbdia, bdia_data);
 SAME_SIZE(clspmv_bcsr_matrix, rbcsr);
@@ -120,6 +123,19 @@ int ref_coo2sbell(clspmv_coo_matrix* s, clspmv_sbell_matrix* d, unsigned bw, uns
     return quiet([&] { return coo2sbell<int, float>(R<rcoo>(s), R<rsbell>(d), bw, bh, sh) ? 1 : 0; });
 }
 
+// X -> COO (the inverses).  csr2coo is left out: its body calls
+// if_sorted_coo on the CSR struct (include/matrix_storage.h:1241-1244), so
+// the template does not compile when instantiated.
+int ref_coo2diaext(clspmv_coo_matrix* s, clspmv_dia_ext_matrix* d, int alignment) {
+    return quiet([&] { return coo2diaext<int, int, float>(R<rcoo>(s), R<rdiaext>(d), alignment) ? 1 : 0; });
+}
+void ref_diaext2coo(clspmv_dia_ext_matrix* s, clspmv_coo_matrix* d) { diaext2coo<int, int, float>(R<rdiaext>(s), R<rcoo>(d)); }
+void ref_dia2coo(clspmv_dia_matrix* s, clspmv_coo_matrix* d) { dia2coo<int, int, float>(R<rdia>(s), R<rcoo>(d)); }
+void ref_bdia2coo(clspmv_bdia_matrix* s, clspmv_coo_matrix* d) { bdia2coo<int, int, float>(R<rbdia>(s), R<rcoo>(d)); }
+void ref_ell2coo(clspmv_ell_matrix* s, clspmv_coo_matrix* d) { ell2coo<int, float>(R<rell>(s), R<rcoo>(d)); }
+void ref_bcsr2coo(clspmv_bcsr_matrix* s, clspmv_coo_matrix* d) { bcsr2coo<int, float>(R<rbcsr>(s), R<rcoo>(d)); }
+void ref_bell2coo(clspmv_bell_matrix* s, clspmv_coo_matrix* d) { bell2coo<int, float>(R<rbell>(s), R<rcoo>(d)); }
+
 void ref_coo_spmv(clspmv_coo_matrix* m, float* x, float* y) { coo_spmv<int, float>(R<rcoo>(m), x, y, m->matinfo.width); }
 void ref_csr_spmv(clspmv_csr_matrix* m, float* x, float* y) { csr_spmv<int, float>(R<rcsr>(m), x, y, m->matinfo.width); }
 void ref_ell_spmv(clspmv_ell_matrix* m, float* x, float* y) { ell_spmv<int, float>(R<rell>(m), x, y, m->matinfo.width); }
@@ -146,5 +162,12 @@ void ref_analyze_matrix(clspmv_coo_matrix* coo, clspmv_cocktail* mat, float mult
 }
 
 void ref_free_cocktail(clspmv_cocktail* mat) { free_cocktail(*R<rcocktail>(mat)); }
+void ref_free_coo(clspmv_coo_matrix* m) {
+    free(m->coo_row_id);
+    free(m->coo_col_id);
+    free(m->coo_data);
+    m->coo_row_id = m->coo_col_id = nullptr;
+    m->coo_data = nullptr;
+}
 
 }  // extern "C"

@@ -10,6 +10,8 @@
  *      the row bound b_i = |y0_i| + sum_j |a_ij x_j| used for the tolerance
  *      |y_i - y64_i| <= tol * b_i.
  *   3. The reference's vector comparison report (src/util.cpp:93-131).
+ *   4. The CSR SpMV of part 1 spread over host threads (OpenMP), for the
+ *      all-core CPU baseline; identical results to the serial routine.
  *
  * How each part is pinned (see tests/test_oracle.py):
  *   - closed forms: 2D/3D Laplacian stencils with x = 1 give y_i = centre -
@@ -45,6 +47,29 @@ void oracle_csr_spmv(int rows, const int* rp, const int* col, const float* val,
         for (int k = rp[r]; k < rp[r + 1]; ++k) acc += val[k] * x[col[k]];
         y[r] += acc;
     }
+}
+
+/* csr_spmv over `threads` host threads (OpenMP, rows split statically):
+ * every row is summed by one thread in the serial loop order, so y is the
+ * same bit for bit as oracle_csr_spmv's.  The all-core CPU baseline. */
+void oracle_csr_spmv_threads(int threads, int rows, const int* rp, const int* col, const float* val,
+                             const float* x, float* y) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int r = 0; r < rows; ++r) {
+        float acc = 0.0f;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) acc += val[k] * x[col[k]];
+        y[r] += acc;
+    }
+}
+
+/* Host threads OpenMP would use by default. */
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+    extern int omp_get_max_threads(void);
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* ell_spmv, include/spmv_serial.h:93 -- slot-major, accumulating into y */

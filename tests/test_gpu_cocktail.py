@@ -103,6 +103,10 @@ def test_profile_restores_y(oracle):
     parts, gms = rt.profile(5)
     assert parts and gms > 0
     np.testing.assert_array_equal(rt.get_y(), before)
+    for graph in (True, False):  # cold-cache timing keeps y too, in both launch modes
+        rt.set_graph(graph)
+        assert rt.time_cold(3, 64 << 20) > 0
+        np.testing.assert_array_equal(rt.get_y(), before)
     rt.close()
 
 
@@ -207,3 +211,40 @@ def test_dia_in_first_pass_launch(oracle):
     rt.close()
     y64, _ = oracle.spmv_f64(D.row, D.col, D.val, xd, D.height, yd)
     np.testing.assert_array_equal(got, y64.astype(np.float32), err_msg=str(modes))
+
+
+def test_failed_tune_leaves_runtime_usable(oracle):
+    """A tuning call that fails part-way (a host-side error injected at trial
+    n, for every n up to the last trial) returns the error, leaves y as it
+    was, and the runtime still computes
+    the exact product (dyadic inputs) in both launch modes: the graph is
+    captured again from the parts' current encodings."""
+    A = _dyadic_copy(cases.matrix("powerlaw_band"))
+    ck = cs.analyze_matrix(A.copy(), os.path.join(GOLD, "ben_dia_first"))
+    assert len(ck.summary()) >= 3, ck.summary()
+    x = matrices.dyadic_vector(A.width, seed=5)
+    y0 = matrices.dyadic_vector(A.height, seed=6)
+    want = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)[0].astype(np.float32)
+    failed = 0
+    for n in range(1, 64):
+        rt = cs.CocktailRuntime(ck, x, y0)
+        os.environ["CLSPMV_TUNE_FAIL_TRIAL"] = str(n)
+        try:
+            rt.tune(1)
+            ok = True
+        except cs.ClspmvError as e:
+            ok = False
+            assert "injected failure" in str(e)
+        finally:
+            del os.environ["CLSPMV_TUNE_FAIL_TRIAL"]
+        np.testing.assert_array_equal(rt.get_y(), y0)
+        failed += not ok
+        for graph in (True, False):
+            rt.set_y(y0)
+            rt.set_graph(graph)
+            rt.do_spmv(1)
+            np.testing.assert_array_equal(rt.get_y(), want, err_msg=f"fail at trial {n}, graph={graph}")
+        rt.close()
+        if ok:
+            break
+    assert failed >= 3

@@ -279,3 +279,19 @@ def test_checks_catch_planted_mistakes(oracle):
     Ad = cs.Coo(A.row[keep], A.col[keep], A.val[keep], A.height, A.width)
     y_d = _oracle_spmv(oracle, Ad, "csr", cs.coo2csr(Ad.copy()), x[:A.width], np.zeros(A.height, np.float32))
     assert not np.array_equal(y_d, want.astype(np.float32))
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_csr_matches_serial(oracle, threads):
+    """The all-core baseline's CSR SpMV sums each row in the serial order:
+    bit-identical to oracle_csr_spmv (and so to the reference's csr_spmv)."""
+    A = matrices.random_sparse(5000, 4000, 0.003, seed=71, long_rows=4, long_len=3000, empty_rows=0.1)
+    csr = cs.coo2csr(A.copy())
+    arr = csr.arrays()
+    x = matrices.uniform_vector(A.width, seed=72)
+    y1 = matrices.uniform_vector(A.height, seed=73)
+    y2 = y1.copy()
+    oracle.csr_spmv(arr["rp"], arr["col"], arr["val"], x, y1)
+    oracle.csr_spmv_threads(threads, arr["rp"], arr["col"], arr["val"], x, y2)
+    np.testing.assert_array_equal(y1, y2)
+    assert oracle.max_threads() >= 1
