@@ -259,6 +259,8 @@ _sig("clspmv_upload_bcsr_lowp", C.c_int, VP, P(BcsrMatrix), C.c_int, P(VP))
 _sig("clspmv_spmv_multi", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int)
 _sig("clspmv_verify_spmv", C.c_int, VP, VP, VP, VP, VP, C.c_double, P(ErrorReport))
 _sig("clspmv_dmat_error_report", C.c_int, VP, VP, VP, C.c_ulonglong, C.c_double, P(ErrorReport))
+_sig("clspmv_verify_spmv_noalloc", C.c_int, VP, VP, VP, VP, VP, C.c_double, VP)
+NOALLOC_MAX_ROW_BLOCKS = 131072
 _sig("clspmv_spmm", C.c_int, VP, VP, C.c_int, VP, C.c_int, VP, C.c_int)
 _sig("clspmv_spmm_tiled", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int)
 _sig("clspmv_spmm_pack_x", C.c_int, VP, C.c_int, C.c_int, VP, C.c_int, VP)
@@ -906,6 +908,24 @@ def verify_spmv(ctx: Context, ref: "DeviceMatrix", x: DeviceVector, y: DeviceVec
     check(lib.clspmv_verify_spmv(ctx.h, ref.h, x.p, y0.p if y0 is not None else None, y.p, tol,
                                  C.byref(rep)), "verify_spmv")
     return rep.as_dict()
+
+
+def verify_spmv_noalloc(ctx: Context, ref: "DeviceMatrix", x: DeviceVector, y: DeviceVector,
+                        report: "DeviceBuffer", y0: Optional[DeviceVector] = None, tol: float = 1e-5) -> None:
+    """clspmv_verify_spmv_noalloc: clspmv_verify_spmv's check without any
+    allocation, asynchronous on the context's stream; the report lands in
+    `report`, a device buffer of at least one clspmv_error_report (read it
+    with report_from_device after a sync)."""
+    if report.nbytes < C.sizeof(ErrorReport):
+        raise ValueError(f"report buffer holds {report.nbytes} bytes, needs {C.sizeof(ErrorReport)}")
+    check(lib.clspmv_verify_spmv_noalloc(ctx.h, ref.h, x.p, y0.p if y0 is not None else None, y.p, tol,
+                                         report.p), "verify_spmv_noalloc")
+
+
+def report_from_device(report: "DeviceBuffer") -> dict:
+    """The clspmv_error_report at the start of a device buffer."""
+    raw = report.download().view(np.uint8)[:C.sizeof(ErrorReport)]
+    return ErrorReport.from_buffer_copy(raw.tobytes()).as_dict()
 
 
 def run_driver(name: str, coo: Coo, ntimes: int = 10, device: int = 0, want_y: bool = True):

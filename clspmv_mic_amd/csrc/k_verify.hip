@@ -14,12 +14,22 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "device_common.hpp"
 #include "kernels.hpp"
 #include "rng.hpp"
 #include "runtime.hpp"
+
+// Per-work-group results of clspmv_verify_spmv_noalloc live in this
+// file-scope array instead of a buffer allocated per call: 32 bytes per CSR
+// row block, one copy per GPU, held until the process ends.  In host code
+// the name `scratch` is a stand-in, not a device address: each GPU's address
+// is looked up once with hipGetSymbolAddress (noalloc_scratch_init, run by
+// clspmv_ctx_create with that GPU current) and only that pointer reaches
+// kernels.
+static __device__ __align__(256) unsigned char scratch[CLSPMV_NOALLOC_MAX_ROW_BLOCKS * 32];
 
 namespace clspmv {
 namespace {
@@ -33,6 +43,11 @@ struct VerifyPart {
     int worst;       // first row reaching max_err
     int over;        // rows above tol
 };
+static_assert(sizeof(VerifyPart) == 32, "scratch holds 32 bytes per row block");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per GPU
 
 __device__ inline double row_error(double y, double y64, double bound) {
     const double d = fabs(y - y64);
@@ -134,6 +149,39 @@ __global__ __launch_bounds__(kBlock) void verify_csr_kernel(
     if (t == 0) part[b] = VerifyPart{s_max[0], s_sq[0], s_abs[0], s_row[0], s_over[0]};
 }
 
+// clspmv_verify_spmv_noalloc: one thread combines the work-groups' results
+// in group (row) order, the order and arithmetic verify_into uses on the
+// host, and writes the report to device memory.
+__global__ void verify_combine_kernel(const VerifyPart* __restrict__ part, int nparts, int rows, double tol,
+                                      clspmv_error_report* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    clspmv_error_report r{};
+    r.rows = rows;
+    r.tol = tol;
+    r.worst_row = -1;
+    if (rows > 0) {
+        double best = -1.0, sq = 0.0, mabs = 0.0;
+        int row = -1;
+        long long over = 0;
+        for (int i = 0; i < nparts; ++i) {
+            const VerifyPart p = part[i];
+            if (p.max_err > best) {
+                best = p.max_err;
+                row = p.worst;
+            }
+            sq += p.sum_sq;
+            mabs = fmax(mabs, p.max_abs);
+            over += p.over;
+        }
+        r.max_scaled_error = best;
+        r.rms_scaled_error = sqrt(sq / rows);
+        r.max_abs_error = mabs;
+        r.worst_row = row;
+        r.rows_over_tol = over;
+    }
+    *out = r;
+}
+
 __global__ void fill_uniform_kernel(unsigned long long seed, int n, float* x) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         x[i] = uniform_at(seed, 0, i);
@@ -146,10 +194,10 @@ __global__ void round16_kernel(int n, bool bf16, const float* __restrict__ x, un
 
 int grid_of(long long n) { return (int)std::min<long long>((n + kBlock - 1) / kBlock, 4096); }
 
-// Device scratch freed on every exit path (after the stream drains).
-struct Scratch {
+// Device temporaries freed on every exit path (after the stream drains).
+struct DeviceTemps {
     std::vector<void*> ptrs;
-    ~Scratch() {
+    ~DeviceTemps() {
         for (void* p : ptrs) (void)hipFree(p);
     }
     template <class T>
@@ -171,7 +219,7 @@ int verify_into(clspmv_ctx* ctx, const clspmv_dmat* ref, const float* d_x, const
     rep->tol = tol;
     rep->worst_row = -1;
     if (I.rows == 0) return CLSPMV_OK;
-    Scratch sc;
+    DeviceTemps sc;
     VerifyPart* d_part = nullptr;
     int st = sc.get(ref->nblocks, &d_part);
     if (st) return st;
@@ -211,7 +259,24 @@ int check_ref(const clspmv_ctx* ctx, const clspmv_dmat* ref) {
     return CLSPMV_OK;
 }
 
+void* scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+
 }  // namespace
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(CLSPMV_ERR_UNSUPPORTED, "device ordinal %d: at most %d devices", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return CLSPMV_OK;
+    void* p = nullptr;
+    CLSPMV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+    return CLSPMV_OK;
+}
+
 }  // namespace clspmv
 
 using namespace clspmv;
@@ -228,6 +293,30 @@ int clspmv_verify_spmv(clspmv_ctx* ctx, const clspmv_dmat* ref, const float* d_x
     return verify_into(ctx, ref, d_x, d_y0, d_y, tol, rep);
 }
 
+int clspmv_verify_spmv_noalloc(clspmv_ctx* ctx, const clspmv_dmat* ref, const float* d_x, const float* d_y0,
+                               const float* d_y, double tol, clspmv_error_report* d_report) {
+    if (!ctx || !ref || !d_report) return fail(CLSPMV_ERR_INVALID, "null argument");
+    int st = check_ref(ctx, ref);
+    if (st) return st;
+    const int rows = ref->info.rows;
+    if (rows > 0 && (!d_y || (ref->info.nnz > 0 && !d_x))) return fail(CLSPMV_ERR_INVALID, "null vector");
+    if (ref->nblocks > CLSPMV_NOALLOC_MAX_ROW_BLOCKS)
+        return fail(CLSPMV_ERR_TOO_LARGE, "the reference has %d row blocks; the no-allocation scratch holds %d",
+                    ref->nblocks, CLSPMV_NOALLOC_MAX_ROW_BLOCKS);
+    if ((st = use_ctx_device(ctx))) return st;
+    VerifyPart* part = static_cast<VerifyPart*>(scratch_of(ctx->device));
+    if (!part) return fail(CLSPMV_ERR_INVALID, "no scratch set up for device %d", ctx->device);
+    if (rows > 0) {
+        hipLaunchKernelGGL(verify_csr_kernel, dim3(ref->nblocks), dim3(kBlock), 0, ctx->stream, ref->blocks,
+                           ref->rp, ref->col, ref->val, d_x, d_y0, d_y, tol, part);
+        CLSPMV_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(verify_combine_kernel, dim3(1), dim3(kWave), 0, ctx->stream, part,
+                       rows > 0 ? ref->nblocks : 0, rows, tol, d_report);
+    CLSPMV_HIP(hipGetLastError());
+    return CLSPMV_OK;
+}
+
 int clspmv_dmat_error_report(clspmv_ctx* ctx, const clspmv_dmat* A, const clspmv_dmat* ref,
                              unsigned long long seed, double tol, clspmv_error_report* rep) {
     if (!ctx || !A || !ref || !rep) return fail(CLSPMV_ERR_INVALID, "null argument");
@@ -240,7 +329,7 @@ int clspmv_dmat_error_report(clspmv_ctx* ctx, const clspmv_dmat* A, const clspmv
     if ((st = use_ctx_device(ctx))) return st;
     const bool block = I.format == CLSPMV_FMT_BSPMM;
     const int ldx = (I.cols + 3) / 4 * 4, ldy = (I.rows + 3) / 4 * 4;
-    Scratch sc;
+    DeviceTemps sc;
     float *x = nullptr, *y = nullptr;
     unsigned short* x16 = nullptr;
     if ((st = sc.get(ldx, &x)) || (st = sc.get(ldy, &y))) return st;
@@ -261,7 +350,7 @@ int clspmv_dmat_error_report(clspmv_ctx* ctx, const clspmv_dmat* A, const clspmv
         st = clspmv_spmv(ctx, A, CLSPMV_METHOD_AUTO, x, y);
     }
     if (!st) st = verify_into(ctx, ref, x, nullptr, y, tol, rep);
-    if (st) (void)hipStreamSynchronize(s);  // before Scratch frees what a launch may still use
+    if (st) (void)hipStreamSynchronize(s);  // before DeviceTemps frees what a launch may still use
     return st;
 }
 

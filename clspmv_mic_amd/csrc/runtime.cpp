@@ -263,6 +263,8 @@ int clspmv_ctx_create(int device_ordinal, clspmv_ctx** out) {
     if (device_ordinal < 0 || device_ordinal >= n)
         return fail(CLSPMV_ERR_INVALID, "device %d not present (%d devices)", device_ordinal, n);
     CLSPMV_HIP(hipSetDevice(device_ordinal));
+    int st = noalloc_scratch_init(device_ordinal);
+    if (st) return st;
     clspmv_ctx* c = new clspmv_ctx();
     c->device = device_ordinal;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -137,6 +137,10 @@ int auto_panel_width(int cols, long long nnz);
 // plain arrays.
 void drop_encoding(clspmv_dmat* d, bool panel);
 
+// Looks up, with `device` current, the device address of the scratch array
+// of clspmv_verify_spmv_noalloc (k_verify.hip); once per device.
+int noalloc_scratch_init(int device);
+
 // Checks that the context's device is current; sets it otherwise.
 int use_ctx_device(clspmv_ctx* ctx);
 

@@ -248,6 +248,23 @@ int clspmv_verify_spmv(clspmv_ctx* ctx, const clspmv_dmat* ref, const float* d_x
 int clspmv_dmat_error_report(clspmv_ctx* ctx, const clspmv_dmat* A, const clspmv_dmat* ref,
                              unsigned long long seed, double tol, clspmv_error_report* rep);
 
+/* clspmv_verify_spmv_noalloc: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): clspmv_verify_spmv's check with no
+ * allocation, no memset and no host synchronisation.  The report (same
+ * values as clspmv_verify_spmv's) is written to d_report, a device pointer
+ * to one clspmv_error_report, asynchronously on the context's stream.
+ * The per-work-group results live in a fixed device array inside the
+ * library, one per GPU, sized for CLSPMV_NOALLOC_MAX_ROW_BLOCKS row blocks
+ * of `ref` (32 bytes each, 4 MiB; a CSR matrix of r rows and z entries has
+ * at most r/256 + 3z/1024 + 1 blocks; the 27-point stencil
+ * on a 128^3 grid has 55,586).  Larger references return CLSPMV_ERR_TOO_LARGE.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (two
+ * contexts' streams, or threads) race and must be ordered by the caller. */
+#define CLSPMV_NOALLOC_MAX_ROW_BLOCKS 131072
+int clspmv_verify_spmv_noalloc(clspmv_ctx* ctx, const clspmv_dmat* ref, const float* d_x, const float* d_y0,
+                               const float* d_y, double tol, clspmv_error_report* d_report);
+
 /* ---- bandwidth probe (src/mem_bandwidth.cpp bandwidth_test :14) ---------
  * Copies `bytes` (rounded down to 16) between two fresh device buffers
  * `ntimes` times after one warm-up copy and returns the average copy time
