@@ -354,35 +354,25 @@ def run_serve(args):
     import numpy as np
     import clspmv_mic_amd as cs
     from clspmv_mic_amd import matrices
-    rng = np.random.default_rng(6)
     nmat = args.matrices
-    sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e6), size=nmat)).astype(np.int64)
     ctx = cs.Context(0)
     server = cs.Server(streams=args.streams, max_batch=64, hbm_budget=16 << 30)
     dev, ids, kinds, errs16 = [], [], [], []
     t0 = time.time()
-    for i in range(nmat):
-        m = int(sizes[i])
-        bf16 = (i // 2) % 2 == 1
+    rng, mix = matrices.serving_mix(nmat, seed=6)
+    for kind, bf16, A in mix:
         dt = cs.DT_BF16 if bf16 else cs.DT_F32
-        if i % 2 == 0:  # stencil: 2D and 3D alternating
-            if (i // 2) % 4 < 2:
-                A = matrices.laplacian_2d(max(8, int(round(m ** 0.5))))
-            else:
-                A = matrices.stencil_27pt(max(4, int(round(m ** (1 / 3)))), seed=100 + i)
+        if kind == "stencil":
             d = cs.DeviceMatrix(ctx, cs.coo2dia(A.copy()), dtype=dt)
-            kind = "stencil"
         else:
-            # CSR with both encodings: single right-hand sides run the
-            # x-panel encoding, several run whichever of k panel SpMVs or one
-            # multi-vector pass the server measured faster.
-            A = matrices.power_law(m, seed=100 + i, band=())
+            # CSR with both encodings: the server runs a batch as one
+            # multi-vector pass per 16 columns (own arrays) or as x-panel
+            # SpMVs per column, whichever measured faster at registration.
             csr = cs.coo2csr(A.copy())
             d = cs.DeviceMatrix(ctx, csr, panel=cs.PANEL_BOTH, dtype=dt)
-            kind = "powerlaw"
         if bf16:  # per-matrix error of the 16-bit storage against FP64 on the GPU
             ref = cs.DeviceMatrix(ctx, csr if kind == "powerlaw" else cs.coo2csr(A.copy()), panel=0)
-            errs16.append(d.error_report(ref, seed=100 + i)["max_scaled_error"])
+            errs16.append(d.error_report(ref, seed=100 + len(dev))["max_scaled_error"])
             del ref
         dev.append(d)
         ids.append(server.add(d, 0, kmax=16))
@@ -515,13 +505,32 @@ def run_sweep(args):
     return 0
 
 
+def run_probe(args):
+    """Bandwidth ceilings of this GPU: the fastest copy and read-only
+    streams at footprints below and above the 256 MB memory-side cache, and
+    random gathers from x vectors of growing size."""
+    import clspmv_mic_amd as cs
+    ctx = cs.Context(0)
+    copy = {f"{mb}MiB": round(ctx.bandwidth_probe(mb << 20, 20)[1], 1) for mb in (64, 256, 1024, 4096)}
+    read = {f"{mb}MiB": round(ctx.read_probe(mb << 20, 20)[1], 1) for mb in (64, 256, 1024, 4096)}
+    gather = {f"nx{nx}": round(ctx.gather_probe(nx, 1 << 25, 10)[1], 1)
+              for nx in (1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24)}
+    out = {"metric": "HBM bandwidth ceilings (copy counts read + write)", "value": max(read.values()),
+           "unit": "GB/s", "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": "probes: float4 copy, float4 read, random gathers"},
+           "copy_gbps": copy, "read_gbps": read, "gather_g_per_s": gather, "device": ctx.info()[0]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "powerlaw", "stencil27", "lap2d", "block4x4", "spmm", "serve", "sweep"],
+                    choices=["headline", "powerlaw", "stencil27", "lap2d", "block4x4", "spmm", "serve", "sweep",
+                             "probe"],
                     help="headline: the 27-point stencil 128^3 with the 1M power-law matrix in the same line")
     ap.add_argument("--dtype", default="bf16", choices=["bf16", "fp16"], help="spmm: value type")
     ap.add_argument("--k", type=int, nargs="+", default=[8, 16, 32], help="spmm: right-hand sides")
@@ -551,6 +560,8 @@ def main():
         return run_serve(args)
     if args.workload == "sweep":
         return run_sweep(args)
+    if args.workload == "probe":
+        return run_probe(args)
 
     names = ["stencil27", "powerlaw"] if args.workload == "headline" else [args.workload]
     results = [measure_cocktail(args, n) for n in names]

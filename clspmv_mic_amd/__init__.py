@@ -278,6 +278,7 @@ _sig("clspmv_spmv", C.c_int, VP, VP, C.c_int, VP, VP)
 _sig("clspmv_time_spmv", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int, C.c_int, P(C.c_double))
 _sig("clspmv_bandwidth_probe", C.c_int, VP, C.c_size_t, C.c_int, P(C.c_double), P(C.c_double))
 _sig("clspmv_gather_probe", C.c_int, VP, C.c_int, C.c_longlong, C.c_int, P(C.c_double), P(C.c_double))
+_sig("clspmv_read_probe", C.c_int, VP, C.c_size_t, C.c_int, P(C.c_double), P(C.c_double))
 DRIVERS = ["csr_scalar", "csr_vector", "coo", "ell", "sell", "dia", "bdia", "bell", "sbell", "bcsr"]
 for _n in DRIVERS:
     _sig(f"clspmv_spmv_{_n}", C.c_int, C.c_char_p, P(CooMatrix), C.c_int, C.c_int, C.c_int,
@@ -554,6 +555,12 @@ class Context:
         """(ms per copy, GB/s counting read + write) of a float4 copy."""
         ms, gbps = C.c_double(), C.c_double()
         check(lib.clspmv_bandwidth_probe(self.h, nbytes, ntimes, C.byref(ms), C.byref(gbps)), "probe")
+        return ms.value, gbps.value
+
+    def read_probe(self, nbytes: int = 1 << 30, ntimes: int = 20):
+        """(ms per pass, GB/s read) of a read-only float4 stream."""
+        ms, gbps = C.c_double(), C.c_double()
+        check(lib.clspmv_read_probe(self.h, nbytes, ntimes, C.byref(ms), C.byref(gbps)), "read_probe")
         return ms.value, gbps.value
 
     def gather_probe(self, nx: int, n: int = 1 << 24, ntimes: int = 20):

@@ -15,6 +15,16 @@
 //                pseudo-random block columns.
 // Each table row holds GFLOP/s = 2 * nnz / time for every method; a method
 // that cannot run on the probe's shape gets 0 and is never chosen.
+//
+// The knobs a table column stands for are tuned inside it: CSR method 0 and
+// BCSR at the fastest lanes-per-row (powers of two around the shape's own
+// count, as the runtime tuner tries them); CSR and COO at the faster of the
+// format's own arrays and the x-panel encoding (the Cocktail runtime
+// uploads both and keeps the faster), the panel probe built on the host
+// for probes of up to kPanelProbeMax entries and its rate carried to the
+// larger dimensions of the same width; SELL's two columns are its two slice
+// heights (32 and 128, the partitioner's sell_meth_num); block shapes are
+// the eight rows of each block table.
 
 #include <algorithm>
 #include <cmath>
@@ -32,7 +42,7 @@ namespace {
 
 using namespace clspmv::dev;
 
-__device__ __forceinline__ unsigned mix(unsigned long long v) {
+__host__ __device__ __forceinline__ unsigned mix(unsigned long long v) {
     v += 0x9E3779B97F4A7C15ull;
     v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
     v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
@@ -286,11 +296,55 @@ int make_block(hipStream_t s, int format, int dim, int bh, int bw, int blocks, c
     return CLSPMV_OK;
 }
 
+constexpr long long kPanelProbeMax = 8LL << 20;
+
+// The x-panel encoding of a CSR / COO probe (`width` entries per row at the
+// columns fill_cols gives with `salt`), built on the host and added to A.
+// False when panels do not pay for this shape (auto_panel_width).
+int add_panel_probe(clspmv_dmat* A, int dim, int width, int salt, bool* made) {
+    *made = false;
+    const long long nnz = (long long)dim * width;
+    const int pw = auto_panel_width(dim, nnz);
+    if (!pw) return CLSPMV_OK;
+    std::vector<int> row(nnz), col(nnz);
+    std::vector<float> val(nnz, 1.0f);
+    for (long long i = 0; i < nnz; ++i) {
+        row[i] = (int)(i / width);
+        col[i] = (int)(mix(((unsigned long long)(i / width) << 20) ^ (unsigned long long)(i % width) ^
+                           ((unsigned long long)salt << 52)) %
+                       (unsigned)dim);
+    }
+    // Entries of a row must be column-sorted for the encoding's row runs.
+    for (int r = 0; r < dim; ++r) std::sort(col.begin() + (long long)r * width, col.begin() + (long long)(r + 1) * width);
+    int st = build_panel(A, dim, dim, nnz, row.data(), col.data(), val.data(), pw, CLSPMV_DT_F32, 0);
+    if (st) return st;
+    *made = true;
+    return CLSPMV_OK;
+}
+
+// Lane counts worth timing around a shape's own (powers of two).
+std::vector<int> lane_sweep(int lanes) {
+    std::vector<int> c;
+    for (int l = std::max(1, lanes / 4); l <= std::min(64, lanes * 4); l <<= 1) c.push_back(l);
+    return c;
+}
+
 struct Timer {
     clspmv_ctx* ctx;
     float* x;
     float* y;
     int reps;
+    // The fastest over `lanes` (A->lanes set to each in turn, then restored).
+    double gflops_lanes(clspmv_dmat* A, int method, long long nnz, const std::vector<int>& lanes) {
+        const int keep = A->lanes;
+        double best = 0.0;
+        for (int l : lanes) {
+            A->lanes = l;
+            best = std::max(best, gflops(A, method, nnz));
+        }
+        A->lanes = keep;
+        return best;
+    }
     // GFLOP/s of `method` on A, 0 if the method cannot run on it.
     double gflops(const clspmv_dmat* A, int method, long long nnz) {
         hipStream_t s = ctx->stream;
@@ -326,10 +380,16 @@ extern "C" int clspmv_autotune(int device, const char* out_dir, int level, int v
     }
     fillf(s, x, kMaxDim + 64, 1.0f);
     fillf(s, y, kMaxDim + 64, 0.0f);
-    Timer T{ctx, x, y, level > 0 ? 20 : 8};
-    const long long cap = level > 0 ? (256LL << 20) : (64LL << 20);  // stored entries per probe
+    // level -1: a reduced grid (three dimensions, 1M entries per probe, 3
+    // launches) for tests of the tables' form.
+    const bool quick = level < 0;
+    Timer T{ctx, x, y, level > 0 ? 20 : quick ? 3 : 8};
+    const long long cap = level > 0 ? (256LL << 20) : quick ? (1LL << 20) : (64LL << 20);  // stored entries per probe
     std::vector<int> dims;
-    for (int d = 1024; d <= kMaxDim; d *= 2) dims.push_back(d);
+    if (quick)
+        dims = {1024, 8192, 65536};
+    else
+        for (int d = 1024; d <= kMaxDim; d *= 2) dims.push_back(d);
     ckt::Tables t;
     auto note = [&](const char* what, int a, int b, const double* gf, int n) {
         if (!verbose) return;
@@ -401,6 +461,16 @@ extern "C" int clspmv_autotune(int device, const char* out_dir, int level, int v
 
     // csr.ben and coo.ben (one work-group count per row: the count the
     // partitioner's load-balance model assumes for these kernels).
+    {
+    // The x-panel rate per width at the largest dimension measured, for the
+    // probes too large to encode on the host.
+    std::vector<std::pair<int, double>> panel_csr, panel_coo;
+    auto carried = [](const std::vector<std::pair<int, double>>& v, int w) {
+        double g = 0.0;
+        for (const auto& p : v)
+            if (p.first == w) g = p.second;
+        return g;
+    };
     for (int d : dims)
         for (int w : {2, 8, 32, 128, 512, 2048}) {
             if ((long long)d * w > cap) break;
@@ -409,17 +479,46 @@ extern "C" int clspmv_autotune(int device, const char* out_dir, int level, int v
             ckt::GroupRow<CLSPMV_CSR_IMP_NUM> rc{(unsigned)d, (unsigned)w, groups, {0}};
             ckt::GroupRow<CLSPMV_COO_IMP_NUM> ro{(unsigned)d, (unsigned)w, groups, {0}};
             clspmv_dmat* A = nullptr;
+            bool panel = false;
             if ((st = make_flat(s, CLSPMV_FMT_CSR, d, w, 0, &A))) goto out;
-            for (int m = 0; m < CLSPMV_CSR_IMP_NUM; ++m) rc.gf[m] = T.gflops(A, m, nnz);
+            rc.gf[0] = T.gflops_lanes(A, 0, nnz, lane_sweep(A->lanes));
+            for (int m = 1; m < CLSPMV_CSR_IMP_NUM; ++m) rc.gf[m] = T.gflops(A, m, nnz);
+            double gp = carried(panel_csr, w);
+            if (nnz <= kPanelProbeMax) {
+                if ((st = add_panel_probe(A, d, w, 3, &panel))) {
+                    done(A);
+                    goto out;
+                }
+                if (panel) {
+                    A->use_panel = true;
+                    gp = T.gflops(A, 0, nnz);
+                    panel_csr.push_back({w, gp});
+                }
+            }
+            for (int m = 0; m < CLSPMV_CSR_IMP_NUM; ++m) rc.gf[m] = std::max(rc.gf[m], gp);
             done(A);
             if ((st = make_flat(s, CLSPMV_FMT_COO, d, w, 0, &A))) goto out;
             ro.gf[0] = T.gflops(A, 0, nnz);
+            gp = carried(panel_coo, w);
+            if (nnz <= kPanelProbeMax) {
+                if ((st = add_panel_probe(A, d, w, 4, &panel))) {
+                    done(A);
+                    goto out;
+                }
+                if (panel) {
+                    A->use_panel = true;
+                    gp = T.gflops(A, 0, nnz);
+                    panel_coo.push_back({w, gp});
+                }
+            }
+            ro.gf[0] = std::max(ro.gf[0], gp);
             done(A);
             t.csr.push_back(rc);
             t.coo.push_back(ro);
             note("csr", d, w, rc.gf, CLSPMV_CSR_IMP_NUM);
             note("coo", d, w, ro.gf, CLSPMV_COO_IMP_NUM);
         }
+    }
 
     // sbell.ben, bell.ben, bcsr.ben: shapes in the partitioner's order.
     {
@@ -440,7 +539,7 @@ extern "C" int clspmv_autotune(int device, const char* out_dir, int level, int v
                     for (int m = 0; m < CLSPMV_BELL_IMP_NUM; ++m) r2.gf[m] = T.gflops(A, m, A->info.nnz);
                     done(A);
                     if ((st = make_block(s, CLSPMV_FMT_BCSR, d, BH[sh], BW[sh], nb, &A))) goto out;
-                    r3.gf[0] = T.gflops(A, 0, A->info.nnz);
+                    r3.gf[0] = T.gflops_lanes(A, 0, A->info.nnz, lane_sweep(A->lanes));
                     done(A);
                     t.sbell.push_back(r1);
                     t.bell.push_back(r2);

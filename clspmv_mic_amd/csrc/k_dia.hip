@@ -60,6 +60,31 @@ __global__ __launch_bounds__(kBlock) void dia1_kernel(
     y[r] += sum;
 }
 
+// One row per lane with every diagonal in flight at once (ndiag <= N): the
+// lane issues all N data loads and x loads before its first product, so a
+// wave keeps N coalesced 256-byte streams outstanding instead of U, and the
+// products are then summed in diagonal order.  Slots j >= ndiag re-load the
+// last diagonal (cache hits) and are not added.
+template <int N, class V>
+__global__ __launch_bounds__(kBlock) void dia_row_kernel(
+    int rows, int cols, int len, int ndiag, const int* __restrict__ off,
+    const typename V::T* __restrict__ data, const float* __restrict__ x,
+    float* __restrict__ y) {
+    const int r = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (r >= rows) return;
+    float d[N], xv[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int jj = j < ndiag ? j : ndiag - 1;  // wave-uniform
+        d[j] = V::ld(data + (size_t)jj * len + r);
+        xv[j] = x_at(x, r + off[jj], cols);
+    }
+    float a = 0.0f;
+#pragma unroll
+    for (int j = 0; j < N; ++j) a = j < ndiag ? fmaf(d[j], xv[j], a) : a;
+    y[r] += a;
+}
+
 // Two consecutive rows per lane (64-bit data loads, len % 2 == 0), U
 // diagonals per iteration.
 template <int U, class V>
@@ -193,10 +218,29 @@ hipError_t dia_typed(hipStream_t s, int rows, int cols, int len, int ndiag,
                      const typename V::T* data, const float* x, float* y) {
     const int span = max_off - min_off;
     switch (method) {
-        case 0:
-            hipLaunchKernelGGL((dia1_kernel<8, V>), dim3(ceil_div(rows, kBlock)), dim3(kBlock), 0, s,
-                               rows, cols, len, ndiag, offsets, data, x, y);
+        case 0: {
+            // Up to 32 diagonals: all in flight (the smallest multiple of 4
+            // slots that holds them); more: 8 at a time.
+            const dim3 g(ceil_div(rows, kBlock)), b(kBlock);
+            switch ((ndiag + 3) / 4) {
+#define CLSPMV_DIA_ROW(Q)                                                                              \
+    case Q:                                                                                           \
+        hipLaunchKernelGGL((dia_row_kernel<4 * Q, V>), g, b, 0, s, rows, cols, len, ndiag, offsets, data, x, y); \
+        break;
+                CLSPMV_DIA_ROW(1)
+                CLSPMV_DIA_ROW(2)
+                CLSPMV_DIA_ROW(3)
+                CLSPMV_DIA_ROW(4)
+                CLSPMV_DIA_ROW(5)
+                CLSPMV_DIA_ROW(6)
+                CLSPMV_DIA_ROW(7)
+                CLSPMV_DIA_ROW(8)
+#undef CLSPMV_DIA_ROW
+                default:
+                    hipLaunchKernelGGL((dia1_kernel<8, V>), g, b, 0, s, rows, cols, len, ndiag, offsets, data, x, y);
+            }
             break;
+        }
         case 1:
             if (len % 2) return hipErrorInvalidValue;
             hipLaunchKernelGGL((dia2_kernel<4, V>), dim3(ceil_div(rows, 2 * kBlock)), dim3(kBlock), 0, s,

@@ -161,6 +161,11 @@ hipError_t pack_x4(hipStream_t s, int k, int cols, const unsigned short* X, int 
 // ---- bandwidth probe ---------------------------------------------------------
 // dst[0..4*n4) = src[0..4*n4) with 128-bit loads/stores (16-byte aligned).
 hipError_t copy4(hipStream_t s, long long n4, const float* src, float* dst);
+// The same copy (unroll = 2, 4 or 8 float4 per thread per step, `grid`
+// work-groups of 256), and a read-only stream writing one float per thread
+// to out (grid * 256 floats).
+hipError_t copy4u(hipStream_t s, long long n4, const float* src, float* dst, int unroll, int grid);
+hipError_t read4u(hipStream_t s, long long n4, const float* src, float* out, int unroll, int grid);
 // y[i] = x[hash(i) % nx] for i < n: random gathers from an nx-float vector.
 hipError_t gather(hipStream_t s, long long n, int nx, const float* x, float* y);
 

@@ -976,6 +976,45 @@ int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
     return CLSPMV_OK;
 }
 
+namespace {
+// (float4 per thread per step, work-groups) of the probes' launch shapes.
+constexpr int kProbeShapes = 5;
+constexpr int kProbeShape[kProbeShapes][2] = {{2, 4096}, {4, 2048}, {4, 4096}, {8, 1024}, {8, 2048}};
+}  // namespace
+
+int clspmv_read_probe(clspmv_ctx* ctx, size_t bytes, int ntimes, double* ms_per_read, double* gbps) {
+    if (!ctx || ntimes <= 0 || bytes < 16) return fail(CLSPMV_ERR_INVALID, "bad argument");
+    int st = use_ctx_device(ctx);
+    if (st) return st;
+    const long long n4 = (long long)(bytes / 16);
+    float *a = nullptr, *out = nullptr;
+    CLSPMV_HIP(hipMalloc(&a, n4 * 16));
+    hipError_t e = hipMalloc(&out, sizeof(float) * 4096 * 256);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return fail(CLSPMV_ERR_ALLOC, "probe buffer: %s", hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(a, 0, n4 * 16, ctx->stream);
+    double per = 1e30;
+    for (int v = 0; v < kProbeShapes && e == hipSuccess; ++v) {
+        auto run = [&]() { return launch::read4u(ctx->stream, n4, a, out, kProbeShape[v][0], kProbeShape[v][1]); };
+        float ms = 0.0f;
+        e = run();
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev_start, ctx->stream);
+        for (int i = 0; i < ntimes && e == hipSuccess; ++i) e = run();
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+        if (e == hipSuccess) per = std::min(per, (double)ms / ntimes);
+    }
+    (void)hipFree(a);
+    (void)hipFree(out);
+    if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "probe failed: %s", hipGetErrorString(e));
+    if (ms_per_read) *ms_per_read = per;
+    if (gbps) *gbps = per > 0 ? (double)n4 * 16 / (per * 1e-3) / 1e9 : 0.0;
+    return CLSPMV_OK;
+}
+
 int clspmv_bandwidth_probe(clspmv_ctx* ctx, size_t bytes, int ntimes,
                            double* ms_per_copy, double* gbps) {
     if (!ctx || ntimes <= 0 || bytes < 16) return fail(CLSPMV_ERR_INVALID, "bad argument");
@@ -989,18 +1028,27 @@ int clspmv_bandwidth_probe(clspmv_ctx* ctx, size_t bytes, int ntimes,
         (void)hipFree(a);
         return fail(CLSPMV_ERR_ALLOC, "probe buffer: %s", hipGetErrorString(e));
     }
-    float ms = 0.0f;
     e = hipMemsetAsync(a, 0, n4 * 16, ctx->stream);
-    if (e == hipSuccess) e = launch::copy4(ctx->stream, n4, a, b);
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev_start, ctx->stream);
-    for (int i = 0; i < ntimes && e == hipSuccess; ++i) e = launch::copy4(ctx->stream, n4, a, b);
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    // The plain one-float4-per-thread copy, then unrolled copies with
+    // non-temporal stores over a few launch shapes; the fastest counts.
+    double per = 1e30;
+    for (int v = 0; v < kProbeShapes + 1 && e == hipSuccess; ++v) {
+        auto run = [&]() {
+            return v == 0 ? launch::copy4(ctx->stream, n4, a, b)
+                          : launch::copy4u(ctx->stream, n4, a, b, kProbeShape[v - 1][0], kProbeShape[v - 1][1]);
+        };
+        float ms = 0.0f;
+        e = run();
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev_start, ctx->stream);
+        for (int i = 0; i < ntimes && e == hipSuccess; ++i) e = run();
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+        if (e == hipSuccess) per = std::min(per, (double)ms / ntimes);
+    }
     (void)hipFree(a);
     (void)hipFree(b);
     if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "probe failed: %s", hipGetErrorString(e));
-    const double per = (double)ms / ntimes;
     if (ms_per_copy) *ms_per_copy = per;
     if (gbps) *gbps = per > 0 ? 2.0 * n4 * 16 / (per * 1e-3) / 1e9 : 0.0;
     return CLSPMV_OK;

@@ -197,6 +197,29 @@ def banded(n: int, offsets, seed: int = 0, dyadic: bool = False) -> Coo:
     return _finish(rows.astype(np.int32), cols.astype(np.int32), vals, n, n)
 
 
+def serving_mix(nmat: int = 64, seed: int = 6):
+    """The serving workload's resident matrices: sizes log-uniform in
+    [10k, 2M] rows; even entries stencils (2D 5-point and 3D 27-point in
+    turn), odd entries power-law matrices without the band; value types
+    FP32 and BF16 alternating in pairs.  Returns (sizes rng, [(kind, bf16,
+    COO)]); the rng continues into the request stream."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e6), size=nmat)).astype(np.int64)
+    out = []
+    for i in range(nmat):
+        m = int(sizes[i])
+        bf16 = (i // 2) % 2 == 1
+        if i % 2 == 0:
+            if (i // 2) % 4 < 2:
+                A = laplacian_2d(max(8, int(round(m ** 0.5))))
+            else:
+                A = stencil_27pt(max(4, int(round(m ** (1 / 3)))), seed=100 + i)
+            out.append(("stencil", bf16, A))
+        else:
+            out.append(("powerlaw", bf16, power_law(m, seed=100 + i, band=())))
+    return rng, out
+
+
 def dyadic_vector(n: int, seed: int = 1) -> np.ndarray:
     """x with entries k/64, |k| <= 64: products with dyadic values k/64 and
     their sums stay exact in FP32 while partial sums are below 2^24/4096."""

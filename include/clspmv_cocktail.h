@@ -93,7 +93,13 @@ void clspmv_set_analyze_log(void (*log)(const char* line));
  * generated in device memory (dimensions 1024 .. 2M rows) and writes the ten
  * tables to out_dir/benchmark/NAME.ben in the reference's grammar, ready for
  * clspmv_analyze_matrix.  level 0: up to 64M stored entries per probe and 8
- * timed launches; level 1: up to 256M entries and 20 launches. */
+ * timed launches; level 1: up to 256M entries and 20 launches; level -1: a
+ * reduced grid (1024, 8192 and 65536 rows, 1M entries, 3 launches) for
+ * checking the tables' form.  Each column holds the best of the knobs it
+ * stands for: CSR method 0 and BCSR over lanes per row (powers of two
+ * around the shape's own), CSR and COO over the format's own arrays and
+ * the x-panel encoding, SELL's two columns are its slice heights 32 and
+ * 128, and the block tables' rows are the eight block shapes. */
 int clspmv_autotune(int device_ordinal, const char* out_dir, int level, int verbose);
 
 /* ---- device runtime, include/spmv_cocktail.h:166-175 ------------------- */

@@ -272,6 +272,14 @@ int clspmv_verify_spmv_noalloc(clspmv_ctx* ctx, const clspmv_dmat* ref, const fl
 int clspmv_bandwidth_probe(clspmv_ctx* ctx, size_t bytes, int ntimes,
                            double* ms_per_copy, double* gbps);
 
+/* Read probe (no counterpart in the reference): streams `bytes` (rounded
+ * down to 16) once per call with 128-bit non-temporal loads, each thread
+ * writing one float; the fastest of a few launch shapes, as GB/s read --
+ * the ceiling of kernels that mostly read.  clspmv_bandwidth_probe also
+ * reports the fastest of its one-float4-per-thread copy and unrolled
+ * copies with non-temporal stores over the same shapes. */
+int clspmv_read_probe(clspmv_ctx* ctx, size_t bytes, int ntimes, double* ms_per_read, double* gbps);
+
 /* Random-gather probe: n reads of x[hash(i) % nx] (no locality) written to
  * a dense y.  Returns the average kernel time and billions of gathers per
  * second: the ceiling of SpMV kernels whose column indices are random. */

@@ -137,7 +137,7 @@ def test_overlapped_first_pass_identical(oracle):
     The pass must give the same y, bit for bit, as the one-stream pass, in
     both launch modes and over repeated passes, and stay within the
     tolerance of the FP64 oracle."""
-    A = matrices.power_law(200000, seed=7)  # DIA (the band) + COO under the shipped tables
+    A = matrices.power_law(500000, seed=7)  # DIA (the band) + COO under the shipped tables
     ck = cs.analyze_matrix(A.copy(), cs.TABLES_DIR)
     summ = ck.summary()
     assert summ.get("dia", 0) > 0 and summ.get("coo", 0) + summ.get("csr", 0) >= 65536, summ
@@ -171,7 +171,7 @@ def test_dia_in_first_pass_launch(oracle):
     plain launches, bit-identical to separate launches when the DIA part
     runs method 1 (whose arithmetic the tail uses), and exact on dyadic
     values."""
-    A = matrices.power_law(200000, seed=7)  # DIA (the band) + COO under the shipped tables
+    A = matrices.power_law(500000, seed=7)  # DIA (the band) + COO under the shipped tables
     ck = cs.analyze_matrix(A.copy(), cs.TABLES_DIR)
     x = matrices.uniform_vector(A.width, seed=41)
     y0 = matrices.uniform_vector(A.height, seed=42)
