@@ -239,8 +239,8 @@ def measure_cocktail(args, name):
         "cold_cache": {"ms_per_step": round(cold_ms, 6),
                        "value": round(2.0 * A.nnz / (cold_ms * 1e-3) / 1e9, 3),
                        "pass_frac": frac(pass_format, cold_ms),
-                       "note": "each pass timed alone after a 512 MiB device copy (1 GiB moved) that "
-                               "evicts the L2s and the 256 MB memory-side cache"},
+                       "note": "each pass timed alone after a 1 GiB read-only stream that evicts the L2s "
+                               "and the 256 MB memory-side cache"},
         "parts": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in p.items()} for p in parts],
         "graph_ms": round(graph_ms, 6),
         "cpu_baseline": cpu,

@@ -22,6 +22,7 @@
 #include <cstdio>
 
 #include "clspmv_cocktail.h"
+#include "kernels.hpp"
 #include "runtime.hpp"
 
 using namespace clspmv;
@@ -434,21 +435,28 @@ int clspmv_cocktail_time_cold(clspmv_cocktail_kernels* k, int ntimes, size_t scr
     if (st) return st;
     clspmv_cocktail_gpu* g = k->gpu;
     hipStream_t s = k->ctx->stream;
-    const size_t half = std::max<size_t>(scrub_bytes / 2 / 256 * 256, 256);
+    // The scrub only reads (a read-only stream over its own buffer), so
+    // the timed pass starts with clean caches full of unrelated lines and no
+    // write-backs pending.
+    const long long n4 = (long long)std::max<size_t>(scrub_bytes / 16, 16);
+    constexpr int kScrubUnroll = 4, kScrubGrid = 2048;
     float* saved = nullptr;
-    char* scrub = nullptr;
+    float* scrub = nullptr;
+    float* sink = nullptr;
     CLSPMV_HIP(hipMalloc(&saved, sizeof(float) * std::max(g->height, 1)));
-    hipError_t e = hipMalloc(&scrub, 2 * half);
+    hipError_t e = hipMalloc(&scrub, (size_t)n4 * 16);
+    if (e == hipSuccess) e = hipMalloc(&sink, sizeof(float) * kScrubGrid * 256);
     if (e != hipSuccess) {
         (void)hipFree(saved);
-        return fail(CLSPMV_ERR_ALLOC, "scrub buffer of %zu bytes: %s", 2 * half, hipGetErrorString(e));
+        if (scrub) (void)hipFree(scrub);
+        return fail(CLSPMV_ERR_ALLOC, "scrub buffer of %zu bytes: %s", (size_t)n4 * 16, hipGetErrorString(e));
     }
-    e = hipMemsetAsync(scrub, 0, 2 * half, s);
+    e = hipMemsetAsync(scrub, 0, (size_t)n4 * 16, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(saved, g->res, sizeof(float) * g->height, hipMemcpyDeviceToDevice, s);
     double total = 0.0;
     for (int r = 0; r < ntimes && e == hipSuccess && !st; ++r) {
-        e = hipMemcpyAsync(scrub + half, scrub, half, hipMemcpyDeviceToDevice, s);
+        e = launch::read4u(s, n4, scrub, sink, kScrubUnroll, kScrubGrid);
         if (e == hipSuccess) e = hipEventRecord(k->ctx->ev_start, s);
         if (e != hipSuccess) break;
         if (k->use_graph)
@@ -465,6 +473,7 @@ int clspmv_cocktail_time_cold(clspmv_cocktail_kernels* k, int ntimes, size_t scr
     (void)hipStreamSynchronize(s);
     (void)hipFree(saved);
     (void)hipFree(scrub);
+    (void)hipFree(sink);
     if (st) return st;
     if (e != hipSuccess || e2 != hipSuccess)
         return fail(CLSPMV_ERR_HIP, "cold timing failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));

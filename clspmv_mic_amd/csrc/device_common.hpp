@@ -89,12 +89,29 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// The lane's value from another lane of its 16-lane row, by DPP (no LDS
+// crossbar round trip): CTRL 0xB1 = quad_perm(1,0,3,2) (lane ^ 1), 0x4E =
+// quad_perm(2,3,0,1) (lane ^ 2), 0x141 = row_half_mirror (lane i <- 7 - i
+// within each 8 lanes), 0x140 = row_mirror (lane i <- 15 - i).
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
 // Sum over aligned groups of G consecutive lanes (G a power of two, at most
-// 64); every lane of a group receives the group's total.
+// 64); every lane of a group receives the group's total (the same bits in
+// every lane: each step adds a pair of values held by both partners).
+// Steps inside a 16-lane row use DPP: after lane ^ 1 and lane ^ 2 each lane
+// holds its quad's sum, the half mirror pairs the two quads of 8 lanes and
+// the row mirror the two halves of 16; larger groups finish with shuffles.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
+    if constexpr (G >= 2) v += dpp_row<0xB1>(v);
+    if constexpr (G >= 4) v += dpp_row<0x4E>(v);
+    if constexpr (G >= 8) v += dpp_row<0x141>(v);
+    if constexpr (G >= 16) v += dpp_row<0x140>(v);
 #pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    for (int m = 16; m < G; m <<= 1) v += __shfl_xor(v, m, kWave);
     return v;
 }
 

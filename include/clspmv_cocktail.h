@@ -161,9 +161,9 @@ int clspmv_cocktail_profile(clspmv_cocktail_kernels* kernels, int ntimes,
                             int* nparts, double* graph_ms);
 
 /* Cold-cache timing (no counterpart in the reference): ntimes passes in the
- * kernels' launch mode, each after a device-to-device copy of scrub_bytes / 2
- * bytes through a buffer of its own (reads and writes scrub_bytes, evicting
- * the L2s and the memory-side cache), each pass alone between HIP events;
+ * kernels' launch mode, each after a read-only stream over a buffer of its
+ * own of scrub_bytes (evicting the L2s and the memory-side cache without
+ * leaving write-backs behind), each pass alone between HIP events;
  * *ms_per_pass is the average.  y is preserved.  Blocks. */
 int clspmv_cocktail_time_cold(clspmv_cocktail_kernels* kernels, int ntimes, size_t scrub_bytes,
                               double* ms_per_pass);
