@@ -31,6 +31,9 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBPS = 8000.0
 
 
+SWEEP_FORMATS = ["csr_scalar", "csr_vector", "coo", "sell", "ell", "dia", "bdia", "bcsr", "bell", "sbell"]
+
+
 def build_matrix(name):
     from clspmv_mic_amd import matrices
     if name == "powerlaw":
@@ -419,6 +422,7 @@ def run_serve(args):
     slo_rate_plain = slo_load(1, plain) * plain
     stats = server.stats()
     server.close()
+    bound = serve_kernel_time(cs, np, ctx, dev, kinds, mat, ks, wall0 / n, wallb / nb, nb)
     served = n / wall
     flops = 2.0 * float(np.sum(nnz_of[mat] * ks))
     out = {
@@ -442,6 +446,7 @@ def run_serve(args):
                 "requests_per_probe": ns},
         "plain_latency_ms_p50": round(float(np.percentile(latb, 50)) * 1e3, 3),
         "server_stats": stats,
+        "kernel_time": bound,
         "gflops_served": round(flops / wall / 1e9, 2),
         "bf16_matrix_error_vs_fp64": {"matrices": len(errs16),
                                       "max": float(f"{max(errs16):.3e}") if errs16 else None,
@@ -451,6 +456,58 @@ def run_serve(args):
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def serve_kernel_time(cs, np, ctx, dev, kinds, mat, ks, server_wall, plain_wall, nb, kmax=16):
+    """Where a request's time goes: per matrix, the HIP-event time of one
+    single-vector SpMV (its best method) and of one multi-vector pass for
+    each k = 1..kmax, summed over the requests of the trace.
+
+    - plain: k single SpMVs per request (what the plain launches run);
+    - best: the cheaper of the two per request (what the server runs);
+    - coalesced: every matrix's columns of the whole trace in full
+      kmax-column passes at the best cost, i.e. the kernel time of a server
+      that batched perfectly.
+    Kernels run one after another here; the server's streams can overlap
+    the passes of small matrices, which none of these sums credit."""
+    nmat = len(dev)
+    rows = max(k[2] for k in kinds)
+    cols = max(d.info.cols for d in dev)
+    dX = cs.DeviceVector(ctx, cols * kmax, np.full(cols * kmax, 0.5, np.float32))
+    dY = cs.DeviceVector(ctx, rows * kmax, np.zeros(rows * kmax, np.float32))
+    t1 = np.zeros(nmat)
+    cost = np.zeros((nmat, kmax + 1))
+    multi_wins = 0
+    for m, d in enumerate(dev):
+        t1[m] = d.time(dX, dY, warmup=2, ntimes=10)
+        for k in range(1, kmax + 1):
+            tm = d.time_multi(k, dX, dY, d.info.rows, warmup=1, ntimes=5)
+            cost[m, k] = min(k * t1[m], tm)
+            multi_wins += tm < k * t1[m]
+    plain = float(np.mean(ks[:nb] * t1[mat[:nb]]))
+    best = float(np.mean(cost[mat, ks]))
+    total = np.bincount(mat, weights=ks, minlength=nmat).astype(np.int64)
+    coal = float(np.sum(total // kmax * cost[:, kmax] + cost[np.arange(nmat), total % kmax]) / len(mat))
+    stencil = np.array([k[0] == "stencil" for k in kinds])
+    per_kind = {}
+    for name, sel in (("stencil", stencil), ("powerlaw", ~stencil)):
+        req = sel[mat]
+        per_kind[name] = {"requests": int(req.sum()),
+                          "plain_kernel_ms": round(float(np.sum(ks[req] * t1[mat[req]])) / len(mat), 5),
+                          "best_kernel_ms": round(float(np.sum(cost[mat[req], ks[req]])) / len(mat), 5)}
+    return {"plain_wall_ms_per_request": round(plain_wall * 1e3, 5),
+            "plain_kernel_ms_per_request": round(plain, 5),
+            "server_wall_ms_per_request": round(server_wall * 1e3, 5),
+            "best_kernel_ms_per_request": round(best, 5),
+            "coalesced_kernel_ms_per_request": round(coal, 5),
+            "capacity_bound_vs_plain": round(plain_wall * 1e3 / coal, 3) if coal > 0 else None,
+            # A request's own kernel time at the best cost: the latency it
+            # has with an idle GPU, whatever the scheduler.
+            "request_kernel_ms": {"p50": round(float(np.percentile(cost[mat, ks], 50)), 5),
+                                  "p99": round(float(np.percentile(cost[mat, ks], 99)), 5),
+                                  "max": round(float(np.max(cost[mat, ks])), 5)},
+            "multi_pass_wins": f"{multi_wins}/{nmat * kmax}",
+            "by_kind_ms_per_request": per_kind}
 
 
 def _driver_quiet(cs, name, coo, ntimes):
@@ -471,22 +528,21 @@ def _driver_quiet(cs, name, coo, ntimes):
 
 
 def run_sweep(args):
-    """Every per-format driver (spmv_<fmt>, include/spmv_<fmt>.h:7) on the
-    power-law 1M + band matrix: one JSON line per format with the best
-    variant's GFLOP/s, or the reason the format was refused."""
+    """The per-format drivers (spmv_<fmt>, include/spmv_<fmt>.h:7; every
+    format by default) on one matrix (the power-law 1M + band matrix by
+    default): one JSON line per format with the best variant's GFLOP/s, or
+    the reason the format was refused."""
     import numpy as np
     import clspmv_mic_amd as cs
-    from clspmv_mic_amd import matrices
-    A = matrices.power_law(1_000_000, seed=2)
+    A, workload = build_matrix(args.matrix)
     longest = int(np.bincount(A.row, minlength=A.height).max())
     ell_entries = longest * ((A.height + 31) // 32 * 32)
-    for name in ["csr_scalar", "csr_vector", "coo", "sell", "ell", "dia", "bdia", "bcsr", "bell", "sbell"]:
+    for name in args.formats:
         out = {"metric": "SpMV GFLOP/s per format (best variant of the format's driver)",
                "value": None, "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": 3,
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                "data": "synthetic",
-               "config": {"workload": "power-law 1M rows, Zipf(1.8) in [1,5000] + band {0,+-1,+-1000}",
-                          "nnz": A.nnz, "format": name}}
+               "config": {"workload": workload, "nnz": A.nnz, "format": name}}
         t0 = time.time()
         if name == "ell" and ell_entries > (1 << 28):
             out["config"]["status"] = (f"refused: ELL pads every row to the longest ({longest}), "
@@ -502,6 +558,52 @@ def run_sweep(args):
                 out["config"]["status"] = f"refused: {e}"
         out["config"]["seconds"] = round(time.time() - t0, 1)
         print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_format(args):
+    """One storage format on its own (clspmv_upload_<fmt>, then clspmv_spmv
+    per method, HIP-event timed) on one matrix: one JSON line per method with
+    GB/s on the bytes the format must move and the row-scaled error of its y
+    against the FP64 SpMV on the GPU."""
+    import numpy as np
+    import clspmv_mic_amd as cs
+    from clspmv_mic_amd import matrices
+    A, workload = build_matrix(args.matrix)
+    fmt, (bh, bw) = args.format, args.block
+    owned = {"csr": lambda: cs.coo2csr(A.copy()), "ell": lambda: cs.coo2ell(A.copy()),
+             "sell": lambda: cs.coo2sell(A.copy()), "dia": lambda: cs.coo2dia(A.copy()),
+             "bdia": lambda: cs.coo2bdia(A.copy()), "bcsr": lambda: cs.coo2b4csr(A.copy(), bw, bh),
+             "bell": lambda: cs.coo2b4ell(A.copy(), bw, bh), "sbell": lambda: cs.coo2sbell(A.copy(), bw, bh)}[fmt]()
+    if owned is None:
+        raise SystemExit(f"{fmt} refused for this matrix")
+    h = owned.header()
+    part = {"format": fmt, "nnz": A.nnz}
+    if fmt == "bcsr":
+        part["block_index_bytes"] = 4 * h["b4csr_block_num"] + 4 * (h["b4csr_row_num"] + 1)
+    elif fmt == "bell":
+        part["block_index_bytes"] = 4 * h["b4ell_row_num"] * h["b4ell_block_num"]
+    elif fmt == "sbell":
+        part["block_index_bytes"] = 4 * int(owned.arrays()["ptr"][-1])
+    nbytes = part_min_bytes(part, A.height, A.width)
+    ctx = cs.Context(0)
+    dA = cs.DeviceMatrix(ctx, owned)
+    ref = cs.DeviceMatrix(ctx, cs.coo2csr(A.copy()), panel=0)
+    x = cs.DeviceVector(ctx, A.width, matrices.uniform_vector(A.width, seed=3))
+    for method in range(dA.methods()):
+        y = cs.DeviceVector(ctx, A.height, np.zeros(A.height, np.float32))
+        dA.spmv(x, y, method)
+        err = cs.verify_spmv(ctx, ref, x, y)["max_scaled_error"]
+        ms = dA.time(x, y, method, warmup=args.warmup, ntimes=args.steps)
+        gbps = nbytes / (ms * 1e-3) / 1e9
+        print(json.dumps({
+            "metric": "SpMV GB/s of one storage format (% roofline)", "value": round(gbps, 2), "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 6),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": workload, "format": fmt, "block": [bh, bw], "method": method, "nnz": A.nnz},
+            "roofline": {"bound": "hbm", "achieved": round(gbps, 2), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                         "frac": round(gbps / PEAK_HBM_GBPS, 4), "traffic": None, "algorithmic_bytes": nbytes},
+            "gflops": round(2.0 * A.nnz / (ms * 1e-3) / 1e9, 2), "error_vs_fp64": err}), flush=True)
     return 0
 
 
@@ -530,7 +632,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "powerlaw", "stencil27", "lap2d", "block4x4", "spmm", "serve", "sweep",
-                             "probe"],
+                             "probe", "format"],
                     help="headline: the 27-point stencil 128^3 with the 1M power-law matrix in the same line")
     ap.add_argument("--dtype", default="bf16", choices=["bf16", "fp16"], help="spmm: value type")
     ap.add_argument("--k", type=int, nargs="+", default=[8, 16, 32], help="spmm: right-hand sides")
@@ -545,6 +647,14 @@ def main():
     ap.add_argument("--slo-ms", type=float, default=2.0, help="serve: p99 latency target")
     ap.add_argument("--slo-requests", type=int, default=4000, help="serve: requests per SLO probe")
     ap.add_argument("--slo-steps", type=int, default=6, help="serve: bisection steps for the SLO rate")
+    ap.add_argument("--matrix", default="powerlaw", choices=["powerlaw", "stencil27", "lap2d", "block4x4"],
+                    help="sweep, format: the matrix")
+    ap.add_argument("--formats", nargs="+", default=SWEEP_FORMATS, choices=SWEEP_FORMATS,
+                    help="sweep: the drivers to run")
+    ap.add_argument("--format", default="bell", choices=["csr", "ell", "sell", "dia", "bdia", "bcsr", "bell", "sbell"],
+                    help="format: the storage format")
+    ap.add_argument("--block", type=int, nargs=2, default=[4, 4], metavar=("BH", "BW"),
+                    help="format: block height and width")
     ap.add_argument("--tables", default=None, help="directory holding benchmark/*.ben")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the tables' methods and graph launches")
@@ -562,6 +672,8 @@ def main():
         return run_sweep(args)
     if args.workload == "probe":
         return run_probe(args)
+    if args.workload == "format":
+        return run_format(args)
 
     names = ["stencil27", "powerlaw"] if args.workload == "headline" else [args.workload]
     results = [measure_cocktail(args, n) for n in names]

@@ -276,6 +276,7 @@ _sig("clspmv_server_replay", C.c_int, VP, C.c_int, VP, C.c_int, P(C.c_double), P
 _sig("clspmv_request_x", None, C.c_ulonglong, C.c_int, C.c_int, PF)
 _sig("clspmv_spmv", C.c_int, VP, VP, C.c_int, VP, VP)
 _sig("clspmv_time_spmv", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int, C.c_int, P(C.c_double))
+_sig("clspmv_time_spmv_multi", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int, C.c_int, C.c_int, P(C.c_double))
 _sig("clspmv_bandwidth_probe", C.c_int, VP, C.c_size_t, C.c_int, P(C.c_double), P(C.c_double))
 _sig("clspmv_gather_probe", C.c_int, VP, C.c_int, C.c_longlong, C.c_int, P(C.c_double), P(C.c_double))
 _sig("clspmv_read_probe", C.c_int, VP, C.c_size_t, C.c_int, P(C.c_double), P(C.c_double))
@@ -864,6 +865,14 @@ class DeviceMatrix:
     def spmv_multi(self, k: int, X, Y, ldy: int):
         """Y[:, q] += A X[:, q] for q < k; X row-major (cols*k), Y column-major."""
         check(lib.clspmv_spmv_multi(self.ctx.h, self.h, k, X.p, Y.p, ldy), "spmv_multi")
+
+    def time_multi(self, k: int, X, Y, ldy: int, warmup: int = 2, ntimes: int = 5) -> float:
+        """ms per spmv_multi pass over k right-hand sides (clspmv_time_spmv_multi;
+        Y keeps accumulating)."""
+        ms = C.c_double()
+        check(lib.clspmv_time_spmv_multi(self.ctx.h, self.h, k, X.p, Y.p, ldy, warmup, ntimes, C.byref(ms)),
+              "time_spmv_multi")
+        return ms.value
 
     def set_panel(self, use: bool):
         check(lib.clspmv_dmat_set_panel(self.h, int(use)), "dmat_set_panel")

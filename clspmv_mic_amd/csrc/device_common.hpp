@@ -98,28 +98,52 @@ __device__ __forceinline__ float dpp_row(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
+// The value of lane ^ M (M a power of two below 64): DPP quad permutes for
+// 1 and 2, a ds_swizzle in bit-mask mode (and-mask 0x1F, xor-mask M: no
+// address computation, unlike ds_bpermute) within each 32 lanes for 4..16,
+// a shuffle across the wave's halves for 32.
+template <int M>
+__device__ __forceinline__ float xor_lane(float v) {
+    static_assert(M >= 1 && M <= 32 && (M & (M - 1)) == 0, "lane xor mask");
+    if constexpr (M == 1) return dpp_row<0xB1>(v);
+    else if constexpr (M == 2) return dpp_row<0x4E>(v);
+    else if constexpr (M < 32)
+        return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1F | (M << 10)));
+    else return __shfl_xor(v, M, kWave);
+}
+
 // Sum over aligned groups of G consecutive lanes (G a power of two, at most
 // 64); every lane of a group receives the group's total (the same bits in
 // every lane: each step adds a pair of values held by both partners).
 // Steps inside a 16-lane row use DPP: after lane ^ 1 and lane ^ 2 each lane
 // holds its quad's sum, the half mirror pairs the two quads of 8 lanes and
-// the row mirror the two halves of 16; larger groups finish with shuffles.
+// the row mirror the two halves of 16; larger groups finish with xor_lane.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
     if constexpr (G >= 2) v += dpp_row<0xB1>(v);
     if constexpr (G >= 4) v += dpp_row<0x4E>(v);
     if constexpr (G >= 8) v += dpp_row<0x141>(v);
     if constexpr (G >= 16) v += dpp_row<0x140>(v);
-#pragma unroll
-    for (int m = 16; m < G; m <<= 1) v += __shfl_xor(v, m, kWave);
+    if constexpr (G >= 32) v += xor_lane<16>(v);
+    if constexpr (G >= 64) v += xor_lane<32>(v);
     return v;
 }
 
-// Same with a run-time group size (uniform across the wave).
-__device__ __forceinline__ float group_sum_rt(float v, int g) {
-    for (int m = g >> 1; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+// Sum over the lanes of an aligned group of g lanes (a power of two, at
+// most 64, uniform across the wave) that share lane % l: v + v(lane ^ m)
+// for m = g/2 down to l, every lane of a set receiving the same bits.
+__device__ __forceinline__ float xor_sum(float v, int g, int l) {
+    if (g >= 64 && l <= 32) v += xor_lane<32>(v);
+    if (g >= 32 && l <= 16) v += xor_lane<16>(v);
+    if (g >= 16 && l <= 8) v += xor_lane<8>(v);
+    if (g >= 8 && l <= 4) v += xor_lane<4>(v);
+    if (g >= 4 && l <= 2) v += xor_lane<2>(v);
+    if (g >= 2 && l <= 1) v += xor_lane<1>(v);
     return v;
 }
+
+// The whole group's sum with a run-time group size.
+__device__ __forceinline__ float group_sum_rt(float v, int g) { return xor_sum(v, g, 1); }
 
 // x[e..e+3] as a float4.  The fast path needs e+3 < n and 16-byte alignment
 // of x (checked once on the host, `x_aligned`); otherwise each element is

@@ -63,27 +63,30 @@ __global__ __launch_bounds__(kBlock) void dia_multi_kernel(
     float acc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = 0.0f;
-#pragma unroll 3
-    for (int j = 0; j < ndiag; ++j) {
-        const float d = V::ld(data + (size_t)j * len + r);
-        const int c = r + off[j];
-        if ((unsigned)c < (unsigned)cols) {
-            float xv[Q];
-            load_xq<Q>(X, c, k, qb, vec, xv);
+    // C diagonals per step, every load of the step issued before its first
+    // product (C * (1 + Q) values in flight per lane); each column is still
+    // summed in diagonal order, one product at a time, skipping columns
+    // outside x.
+    constexpr int C = Q == 4 ? 8 : 16;
+    for (int j0 = 0; j0 < ndiag; j0 += C) {  // wave-uniform
+        float d[C], xv[C][Q];
+        bool in[C];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) acc[q] = fmaf(d, xv[q], acc[q]);
+        for (int u = 0; u < C; ++u) {
+            const int j = min(j0 + u, ndiag - 1);
+            d[u] = V::ld(data + (size_t)j * len + r);
+            const int c = r + off[j];
+            in[u] = j0 + u < ndiag && (unsigned)c < (unsigned)cols;
+            load_xq<Q>(X, in[u] ? c : 0, k, qb, vec, xv[u]);
         }
+#pragma unroll
+        for (int u = 0; u < C; ++u)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[q] = in[u] ? fmaf(d[u], xv[u][q], acc[q]) : acc[q];
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         if (qb + q < k) Y[(size_t)(qb + q) * ldy + r] += acc[q];
-}
-
-// Sum over the lanes of a group of G (a power of two <= 64) lanes that
-// share lane % L: xor steps G/2 .. L.
-__device__ __forceinline__ float quad_sum(float v, int g, int l) {
-    for (int m = g >> 1; m >= l; m >>= 1) v += __shfl_xor(v, m, kWave);
-    return v;
 }
 
 // Accumulates a[q] += val[j] * X[col[j]][qb + q] over j = j0, j0 + step, ...
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
         for (int q = 0; q < Q; ++q) a[q] = 0.0f;
         csr_multi_run<Q, V, float>(col, val, rp[r0 + r] + lane / L, rp[r0 + r + 1], g / L, X, k, qb, vec, a);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) a[q] = quad_sum(a[q], g, L);
+        for (int q = 0; q < Q; ++q) a[q] = xor_sum(a[q], g, L);
         if (lane < L) {
 #pragma unroll
             for (int q = 0; q < Q; ++q)

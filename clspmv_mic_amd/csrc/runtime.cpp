@@ -976,6 +976,26 @@ int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
     return CLSPMV_OK;
 }
 
+int clspmv_time_spmv_multi(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const float* d_X, float* d_Y,
+                           int ldy, int warmup, int ntimes, double* ms_per_call) {
+    if (!ctx || !A || ntimes <= 0 || warmup < 0) return fail(CLSPMV_ERR_INVALID, "bad argument");
+    for (int i = 0; i < warmup; ++i) {
+        int st = clspmv_spmv_multi(ctx, A, k, d_X, d_Y, ldy);
+        if (st) return st;
+    }
+    CLSPMV_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    for (int i = 0; i < ntimes; ++i) {
+        int st = clspmv_spmv_multi(ctx, A, k, d_X, d_Y, ldy);
+        if (st) return st;
+    }
+    CLSPMV_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    CLSPMV_HIP(hipEventSynchronize(ctx->ev_stop));
+    float ms = 0.0f;
+    CLSPMV_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (ms_per_call) *ms_per_call = (double)ms / ntimes;
+    return CLSPMV_OK;
+}
+
 namespace {
 // (float4 per thread per step, work-groups) of the probes' launch shapes.
 constexpr int kProbeShapes = 5;

@@ -178,6 +178,12 @@ int clspmv_time_spmv(clspmv_ctx* ctx, const clspmv_dmat* A, int method,
                      const float* d_x, float* d_y, int warmup, int ntimes,
                      double* ms_per_call);
 
+/* The same for clspmv_spmv_multi over k right-hand sides (no counterpart in
+ * the reference): ntimes back-to-back passes after warmup untimed ones, HIP
+ * events on the context's stream, Y accumulating.  Blocks. */
+int clspmv_time_spmv_multi(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const float* d_X,
+                           float* d_Y, int ldy, int warmup, int ntimes, double* ms_per_call);
+
 /* ---- mixed-precision block SpMM (no counterpart in the reference) -------
  * Y += A X for k right-hand sides with A's values stored in BF16 or FP16
  * and the products accumulated in FP32 on the matrix cores (MFMA 4x4x4: one

@@ -438,3 +438,26 @@ def test_spmv_multi_refuses_other_formats(ctx):
     dY = cs.DeviceVector(ctx, A.height * 2)
     with pytest.raises(cs.ClspmvError):
         dA.spmv_multi(2, dX, dY, A.height)
+
+
+@pytest.mark.parametrize("fmt", ["dia", "csr"])
+def test_time_spmv_multi_accumulates(ctx, oracle, fmt):
+    """clspmv_time_spmv_multi runs warmup + ntimes passes into the same Y: on
+    dyadic data Y = Y0 + (warmup + ntimes) A X exactly, and the time is
+    positive."""
+    A = CASES["stencil27_dyadic"]
+    owned = cs.coo2dia(A) if fmt == "dia" else cs.coo2csr(A.copy())
+    k, warmup, ntimes = 5, 2, 3
+    rng = np.random.default_rng(8)
+    X = (rng.integers(-8, 9, size=(A.width, k)) / 8.0).astype(np.float32)
+    Y0 = (rng.integers(-8, 9, size=(k, A.height)) / 8.0).astype(np.float32)
+    dA = cs.DeviceMatrix(ctx, owned, panel=0 if fmt == "csr" else None)
+    dY = cs.DeviceVector(ctx, Y0.size, Y0.reshape(-1))
+    ms = dA.time_multi(k, cs.DeviceVector(ctx, X.size, X.reshape(-1)), dY, A.height, warmup, ntimes)
+    assert ms > 0
+    Y = dY.download().reshape(k, A.height)
+    for q in range(k):
+        y64, _ = oracle.spmv_f64(A.row, A.col, A.val, X[:, q].copy(), A.height)
+        np.testing.assert_array_equal(Y[q], (Y0[q] + (warmup + ntimes) * y64).astype(np.float32))
+    with pytest.raises(cs.ClspmvError):
+        dA.time_multi(k, cs.DeviceVector(ctx, X.size, X.reshape(-1)), dY, A.height, 0, 0)
