@@ -71,12 +71,17 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
 // serves two of them when the stride is even (X16); a right-hand side past
 // k reads some in-bounds tile of the chunk -- its column of the product is
 // never stored, and an MFMA output column depends only on its own B column.
-template <bool BF16, int NQ, bool LONG, bool TILED, bool X16>
+// NT (TILED, one pass of at most 4 * NQ right-hand sides): A and its block
+// columns are read once, with non-temporal loads, so they do not evict the
+// X tiles that neighbouring slices re-read; with several passes A must stay
+// cached for the later ones and is loaded normally.
+template <bool BF16, int NQ, bool LONG, bool TILED, bool X16, bool NT>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int brows, int rows, int cols, int kall, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
     const unsigned short* __restrict__ Xall, int ldx, float* __restrict__ Yall, int ldy) {
     static_assert(!X16 || (TILED && NQ >= 2), "16-byte loads take two tiles of the tiled layout");
+    static_assert(!NT || TILED, "non-temporal A loads belong to the tiled kernel");
     static_assert(launch::kSpmmSlice * 4 == kWave, "a wave holds one slice, four lanes per block row");
     const int lane = threadIdx.x & (kWave - 1);
     const int t = lane & 3, g = lane >> 2;
@@ -146,8 +151,15 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
                 auto slot = [&](int s) {
                     return (size_t)base + (size_t)min(s, nmax - 1) * launch::kSpmmSlice + g;
                 };
-                auto ld_col = [&](int s) { return bcol[slot(s)]; };
-                auto ld_a = [&](int s) { return *reinterpret_cast<const s16x4*>(val + slot(s) * 16 + t * 4); };
+                auto ld_col = [&](int s) {
+                    if constexpr (NT) return __builtin_nontemporal_load(bcol + slot(s));
+                    else return bcol[slot(s)];
+                };
+                auto ld_a = [&](int s) {
+                    const s16x4* p = reinterpret_cast<const s16x4*>(val + slot(s) * 16 + t * 4);
+                    if constexpr (NT) return __builtin_nontemporal_load(p);
+                    else return *p;
+                };
                 auto ld_x = [&](int c0, s16x4* b) {
                     // Padding slots (c0 < 0) read tile 0 and are zeroed in mma.
                     const unsigned short* xt = X + (size_t)(c0 >= 0 ? c0 >> 2 : 0) * ldx * 4;
@@ -245,11 +257,26 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
         // 4 * NQ right-hand sides at a time over this slice: the later
         // passes find the slice's blocks and column indices in cache, so A
         // is read from memory about once whatever k is.
+        if constexpr (NT) {
+            run(Xall, Yall, kall);  // kall <= 4 * NQ
+            return;
+        }
         for (int c = 0; c < kall; c += 4 * NQ)
             run(Xall + (size_t)c * 4, Yall + (size_t)c * ldy, min(4 * NQ, kall - c));
     } else {
         run(Xall, Yall, kall);
     }
+}
+
+template <bool BF16, int NQ, bool LONG, bool TILED, bool X16>
+void launch_kernel(hipStream_t s, unsigned blocks, bool nt, int brows, int rows, int cols, int k, const int* rp,
+                   const int* bcol, const unsigned short* val, const unsigned short* X, int ldx, float* Y, int ldy) {
+    if (TILED && nt)
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, TILED>), dim3(blocks), dim3(kBlock), 0, s,
+                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+    else
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, false>), dim3(blocks), dim3(kBlock), 0, s,
+                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
 }
 
 template <bool BF16, int NQ, bool TILED>
@@ -264,20 +291,23 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
     // of them to load.
     constexpr bool kX16 = TILED && NQ >= 2;
     const bool x16 = kX16 && ldx % 2 == 0 && k % 2 == 0 && ((uintptr_t)X & 15) == 0;
+    // One pass over A (k <= 4 * NQ): non-temporal A loads.  Measured on the
+    // 524,288 x 12 4x4-block matrix: k = 8 / 16 in 76.6 / 122 us against
+    // 78.5 / 130 us with normal loads; at k = 32 (two passes) normal loads
+    // keep A in cache for the second pass (247 us against 262 us).
+    const bool nt = k <= 4 * NQ;
+    const unsigned nb = (unsigned)blocks;
     if (long_rows) {
         if (x16)
-            hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, true, TILED, kX16>), dim3((unsigned)blocks), dim3(kBlock), 0,
-                               s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+            launch_kernel<BF16, NQ, true, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
         else
-            hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, true, TILED, false>), dim3((unsigned)blocks), dim3(kBlock),
-                               0, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+            launch_kernel<BF16, NQ, true, TILED, false>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
     } else {
         if (x16)
-            hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, false, TILED, kX16>), dim3((unsigned)blocks), dim3(kBlock),
-                               0, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+            launch_kernel<BF16, NQ, false, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
         else
-            hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, false, TILED, false>), dim3((unsigned)blocks), dim3(kBlock),
-                               0, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+            launch_kernel<BF16, NQ, false, TILED, false>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y,
+                                                         ldy);
     }
     return hipGetLastError();
 }
