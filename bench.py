@@ -337,7 +337,10 @@ def run_spmm(args):
                        "x_layout": "4-row tiles" if tiled else "column-major"},
             "roofline": {"bound": "hbm", "achieved": round(nbytes / (ms * 1e-3) / 1e9, 2), "peak": PEAK_HBM_GBPS,
                          "unit": "GB/s", "frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
-                         "traffic": None, "algorithmic_bytes": nbytes},
+                         # counters of the BF16 tiled kernel at this k, where measured
+                         "traffic": measured_traffic(f"spmm-k{k}", "bspmm4_kernel")
+                         if tiled and args.dtype == "bf16" else None,
+                         "algorithmic_bytes": nbytes},
             "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
             "column_major_x_ms": round(other_ms, 6) if other_ms is not None else None,
             "error_vs_fp64_stored_values": [float(f"{e:.3e}") for e in errs],
