@@ -146,7 +146,7 @@ def measured_traffic(workload, kernel_prefix):
 
 
 # Kernel-name prefix of each format's dominant kernel (rocprofv3 names).
-KERNEL_OF = {"dia": "dia", "bdia": "bdia", "coo": "panel_pass", "csr": "panel_pass", "bcsr": "bcsr_kernel",
+KERNEL_OF = {"dia": "dia", "bdia": "bdia", "coo": "panel_pass", "csr": "panel_pass", "bcsr": "bcsr",
              "bell": "bell_kernel", "sbell": "sbell_kernel", "ell": "ell", "sell": "sell_kernel"}
 
 

@@ -154,7 +154,8 @@ class DmatInfo(C.Structure):
                 ("stored", C.c_longlong), ("device_bytes", C.c_size_t), ("bheight", C.c_int),
                 ("bwidth", C.c_int), ("slice_height", C.c_int), ("panel_width", C.c_int),
                 ("panel_active", C.c_int), ("plain", C.c_int), ("dtype", C.c_int),
-                ("panel_groups", C.c_int), ("panel_subchunks", C.c_int), ("panel_blocks", C.c_int)]
+                ("panel_groups", C.c_int), ("panel_subchunks", C.c_int), ("panel_blocks", C.c_int),
+                ("block_slots", C.c_int)]
 
 
 class DriverReport(C.Structure):

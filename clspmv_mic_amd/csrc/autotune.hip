@@ -251,14 +251,14 @@ int make_block(hipStream_t s, int format, int dim, int bh, int bw, int blocks, c
     };
     switch (format) {
         case CLSPMV_FMT_BCSR: {
-            m->aligned = round_up((int)(4 * nblk), CLSPMV_GPU_ALIGNMENT);
-            m->lanes = 1;
-            while (m->lanes < blocks && m->lanes < 64) m->lanes <<= 1;
-            st = b.alloc(nbr + 1LL, &m->rp);
-            if (!st) st = fillr(s, m->rp, nbr + 1LL, blocks);
-            if (!st) st = b.alloc(nblk, &m->col);
-            if (!st) st = cols(m->col, nblk, blocks);
-            const long long nd = (long long)m->aligned * bh * bw4;
+            // Every block row holds `blocks` blocks: the upload would pick the
+            // slot-major encoding (no padding), so the probe times that.
+            m->hal = round_up(nbr, 64);
+            m->bslots = blocks;
+            const long long nc = (long long)m->hal * blocks;
+            st = b.alloc(nc, &m->col);
+            if (!st) st = cols(m->col, nc, 1);
+            const long long nd = nc * bh * bw4 * 4;
             if (!st) st = b.alloc(nd, &m->val);
             if (!st) st = fillf(s, m->val, nd, 1.0f);
             break;

@@ -526,7 +526,8 @@ int tune_parts(clspmv_cocktail_kernels* k, int ntimes) {
         double best = 1e30;
         int best_m = g->methods[i], best_l = A->lanes;
         bool best_panel = A->use_panel;
-        const bool lane_fmt = fmt == CLSPMV_FMT_CSR || fmt == CLSPMV_FMT_BCSR;
+        // Lanes per row: CSR method 0, BCSR in its row-pointer encoding.
+        const bool lane_fmt = fmt == CLSPMV_FMT_CSR || (fmt == CLSPMV_FMT_BCSR && A->bslots == 0);
         const std::vector<int> lanes = lane_fmt ? lane_candidates(A->lanes) : std::vector<int>{A->lanes};
         // The part's state before its trials, restored if one fails.
         const int upload_lanes = A->lanes, upload_method = g->methods[i];
@@ -561,7 +562,7 @@ int tune_parts(clspmv_cocktail_kernels* k, int ntimes) {
             st = trial(0, upload_lanes, true);
         for (int m = 0; !st && A->plain && m < clspmv_method_count(fmt); ++m) {
             if (!method_runs(A, m)) continue;
-            const bool uses_lanes = (fmt == CLSPMV_FMT_CSR && m == 0) || fmt == CLSPMV_FMT_BCSR;
+            const bool uses_lanes = lane_fmt && (fmt == CLSPMV_FMT_BCSR || m == 0);
             for (int l : uses_lanes ? lanes : std::vector<int>{upload_lanes})
                 if ((st = trial(m, l, false))) break;
         }

@@ -57,6 +57,43 @@ __global__ __launch_bounds__(kBlock) void bcsr_kernel(
     }
 }
 
+// BCSR in the slot-major encoding: one lane per block row walks its slots,
+// each slot's block columns and value planes contiguous across the wave
+// (BELL's pattern).  Padding slots (column -1) hold zero values; their loads
+// go to block column 0 and their products are not added, so the sum is the
+// row's blocks' in block order, one float4 dot per (w4, h).
+template <int BH, int BW4>
+__global__ __launch_bounds__(kBlock) void bcsr_slots_kernel(
+    int brows, int rows, int cols, int hal, int nslots, const int* __restrict__ col4,
+    const float* __restrict__ data, const float* __restrict__ x, bool x_aligned, float* __restrict__ y) {
+    const int r = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (r >= brows) return;
+    float acc[BH];
+#pragma unroll
+    for (int h = 0; h < BH; ++h) acc[h] = 0.0f;
+    const size_t plane = (size_t)hal * 4;  // floats per (slot, w4, h) plane
+    for (int sl = 0; sl < nslots; ++sl) {
+        const int c4 = ld_stream(col4 + (size_t)sl * hal + r);
+        const bool live = c4 >= 0;
+        const float* base = data + (size_t)sl * BW4 * BH * plane + (size_t)r * 4;
+#pragma unroll
+        for (int w4 = 0; w4 < BW4; ++w4) {
+            const float4 xv = ld_x4(x, ((live ? c4 : 0) + w4) * 4, cols, x_aligned);
+#pragma unroll
+            for (int h = 0; h < BH; ++h) {
+                const float4 d = ld_stream(reinterpret_cast<const float4*>(base + (w4 * BH + h) * plane));
+                const float a = dot4(d, xv, acc[h]);
+                acc[h] = live ? a : acc[h];
+            }
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < BH; ++h) {
+        const int row = r * BH + h;
+        if (row < rows) y[row] += acc[h];
+    }
+}
+
 // F4ACC = false: one scalar accumulator per block row row (method 0);
 // true: float4 accumulators reduced at the end (method 1, the reference's
 // "_mad" variant, kernels/spmv_bell.cl:24-42).
@@ -185,6 +222,21 @@ hipError_t bcsr(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
     if (brows <= 0) return hipSuccess;
     SHAPE_DISPATCH(bcsr_shape, s, brows, rows, cols, lanes, aligned, rp, col4,
                    data, x, x_aligned, y)
+}
+
+template <int BH, int BW4>
+static hipError_t bcsr_slots_shape(hipStream_t s, int brows, int rows, int cols, int hal, int nslots,
+                                   const int* col4, const float* data, const float* x, bool xa, float* y) {
+    hipLaunchKernelGGL((bcsr_slots_kernel<BH, BW4>), dim3(grid_for(brows)), dim3(kBlock), 0, s, brows, rows, cols,
+                       hal, nslots, col4, data, x, xa, y);
+    return hipGetLastError();
+}
+
+hipError_t bcsr_slots(hipStream_t s, int brows, int rows, int cols, int bh, int bw, int hal, int nslots,
+                      const int* col4, const float* data, const float* x, bool x_aligned, float* y) {
+    if (brows <= 0 || nslots <= 0) return hipSuccess;
+    if (hal < brows || nslots > kBcsrSlotsMax) return hipErrorInvalidValue;
+    SHAPE_DISPATCH(bcsr_slots_shape, s, brows, rows, cols, hal, nslots, col4, data, x, x_aligned, y)
 }
 
 template <int BH, int BW4>

@@ -118,6 +118,12 @@ hipError_t bdia(hipStream_t s, int rows, int cols, int len, int nband,
 hipError_t bcsr(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
                 int lanes, int aligned, const int* rp, const int* col4,
                 const float* data, const float* x, bool x_aligned, float* y);
+// BCSR in the slot-major encoding: block rows of at most kBcsrSlotsMax
+// slots (the sums are FP32 chains of at most 64 * 8 products per row).
+constexpr int kBcsrSlotsMax = 64;
+hipError_t bcsr_slots(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
+                      int hal, int nslots, const int* col4, const float* data,
+                      const float* x, bool x_aligned, float* y);
 hipError_t bell(hipStream_t s, int brows, int rows, int cols, int bh, int bw,
                 int method, int hal, int f4, int ellnum, const int* col4,
                 const float* data, const float* x, bool x_aligned, float* y);

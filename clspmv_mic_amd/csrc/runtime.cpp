@@ -210,8 +210,12 @@ int dmat_launch_rest(hipStream_t s, const clspmv_dmat* A, int method,
             break;
         }
         case CLSPMV_FMT_BCSR:
-            e = launch::bcsr(s, A->brows, I.rows, I.cols, I.bheight, I.bwidth, A->lanes,
-                             A->aligned, A->rp, A->col, A->val, x, xa, y);
+            if (A->bslots > 0)
+                e = launch::bcsr_slots(s, A->brows, I.rows, I.cols, I.bheight, I.bwidth, A->hal, A->bslots,
+                                       A->col, A->val, x, xa, y);
+            else
+                e = launch::bcsr(s, A->brows, I.rows, I.cols, I.bheight, I.bwidth, A->lanes,
+                                 A->aligned, A->rp, A->col, A->val, x, xa, y);
             break;
         case CLSPMV_FMT_BELL:
             e = launch::bell(s, A->brows, I.rows, I.cols, I.bheight, I.bwidth, method,
@@ -653,6 +657,42 @@ int clspmv_upload_b4csr(clspmv_ctx* ctx, const clspmv_b4csr_matrix* m, clspmv_dm
     d->info.stored = (long long)nb * bh * bw;
     double avg = nbr ? (double)nb / nbr : 0.0;
     d->lanes = pow2_at_least((int)std::ceil(avg), 64);
+    const int* rp = m->b4csr_row_ptr;
+    int w = 0;
+    for (int r = 0; r < nbr; ++r) {
+        if (rp[r + 1] < rp[r]) UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "b4csr row pointers decrease at %d", r));
+        w = std::max(w, rp[r + 1] - rp[r]);
+    }
+    if (rp[0] != 0 || rp[nbr] != nb) UPLOAD_CHECK(fail(CLSPMV_ERR_INVALID, "b4csr row pointers do not span the blocks"));
+    // Slot-major encoding when every block row fits kBcsrSlotsMax slots and
+    // padding the rows to the longest adds at most 10 % to the blocks: a
+    // lane per block row then reads each slot's columns and values as
+    // contiguous runs across the wave, as BELL does.  The row-pointer
+    // arrays are not uploaded.
+    if (nb > 0 && w <= launch::kBcsrSlotsMax && (long long)w * nbr * 10 <= (long long)nb * 11) {
+        const int bw4 = (int)bw / 4;
+        const long long hal = ((long long)nbr + 63) / 64 * 64;
+        std::vector<int> scol((size_t)(hal * w), -1);
+        std::vector<float> sval((size_t)(hal * w * bw4 * bh * 4), 0.0f);
+        const size_t plane = (size_t)m->b4csr_aligned_size * bh;
+        for (int r = 0; r < nbr; ++r)
+            for (int k = 0; k < rp[r + 1] - rp[r]; ++k) {
+                const int j = rp[r] + k;
+                scol[(size_t)k * hal + r] = m->b4csr_col_id[j];
+                for (int w4 = 0; w4 < bw4; ++w4)
+                    for (unsigned h = 0; h < bh; ++h) {
+                        const float* src = m->b4csr_data + w4 * plane + (size_t)h * m->b4csr_aligned_size + (size_t)j * 4;
+                        float* dst = sval.data() + (((size_t)k * bw4 + w4) * bh + h) * hal * 4 + (size_t)r * 4;
+                        std::memcpy(dst, src, sizeof(float) * 4);
+                    }
+            }
+        d->hal = (int)hal;
+        d->bslots = w;
+        UPLOAD_CHECK(upload_array(d, scol.data(), scol.size(), &d->col));
+        UPLOAD_CHECK(upload_array(d, sval.data(), sval.size(), &d->val));
+        *out = d;
+        return CLSPMV_OK;
+    }
     UPLOAD_CHECK(upload_array(d, m->b4csr_row_ptr, (size_t)nbr + 1, &d->rp));
     UPLOAD_CHECK(upload_array(d, m->b4csr_col_id, (size_t)nb, &d->col));
     UPLOAD_CHECK(upload_array(d, m->b4csr_data, (size_t)m->b4csr_aligned_size * bh * (bw / 4), &d->val));
@@ -930,6 +970,7 @@ int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out) {
     out->panel_blocks = m->pw ? m->pnb : 0;
     out->plain = m->plain ? 1 : 0;
     out->dtype = m->dtype;
+    out->block_slots = m->bslots;
     return CLSPMV_OK;
 }
 

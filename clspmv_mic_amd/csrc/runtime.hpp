@@ -42,6 +42,10 @@ struct clspmv_dmat {
     int len = 0, ndiag = 0, nband = 0, min_off = 0, max_off = 0, max_band = 0;
     // Block formats: block rows, aligned sizes, group lanes.
     int brows = 0, aligned = 0, f4 = 0;
+    // BCSR in the slot-major encoding (bslots > 0, rp == nullptr): block
+    // row r's k-th block column at col[k * hal + r] (-1 past the row's end),
+    // its (w4, h) float4 at val[((k * bw4 + w4) * bh + h) * hal * 4 + r * 4].
+    int bslots = 0;
     bool long_brows = false;  // BSPMM: a block row holds more than kLongRowF64 / 4 blocks
     // x-panel encoding of COO / CSR (pw == 0: absent).  plain == false when
     // the (row, col, val) / CSR arrays were not uploaded.

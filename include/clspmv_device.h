@@ -67,7 +67,8 @@ enum clspmv_format {
  *   BDIA 0: row per lane              1: row per lane, LDS x window per band
  *        2: two rows per lane, LDS    3: two rows per lane (64-bit loads)
  *        4: four rows per lane, LDS x window per band
- *   BCSR 0: lane group per block row (group size from blocks per row)
+ *   BCSR 0: lane group per block row (group size from blocks per row),
+ *           or, in the slot-major encoding, one lane per block row
  *   BELL 0: lane per block row, scalar FMA   1: float4 accumulators
  *   SBELL 0: lane per block row within a slice                       */
 #define CLSPMV_METHOD_AUTO (-1)
@@ -90,6 +91,8 @@ typedef struct {
     int    panel_groups;    /* panel encoding: pass-1 work-groups        */
     int    panel_subchunks; /* panel encoding: 512-entry sub-chunks      */
     int    panel_blocks;    /* panel encoding: pass-2 row blocks         */
+    int    block_slots;     /* BCSR slot-major encoding: blocks per block
+                               row after padding, or 0 (row pointers)     */
 } clspmv_dmat_info;
 
 /* Upload a host matrix (the reference's cpy_<fmt>_mat,

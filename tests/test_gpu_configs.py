@@ -137,8 +137,28 @@ def blocks4x4():
 
 
 def test_block4x4_524288_cocktail(oracle, blocks4x4):
+    # One block format takes the whole matrix: BCSR or BELL, whichever the
+    # tables rate faster at 12 blocks per block row (both run a lane per
+    # block row over contiguous slot planes here, within noise of each other).
     ck = _cocktail(oracle, blocks4x4, False, "4x4 blocks 524,288 x 12")
-    assert "bcsr" in ck.summary(), ck.summary()
+    summ = ck.summary()
+    assert len(summ) == 1 and set(summ) <= {"bcsr", "bell"}, summ
+
+
+def test_block4x4_524288_bcsr(ctx, oracle, blocks4x4):
+    """The matrix as BCSR on its own (clspmv_upload_b4csr): the slot-major
+    encoding, 12 slots per block row and no padding, against the oracle."""
+    A = blocks4x4
+    dA = cs.DeviceMatrix(ctx, cs.coo2b4csr(A.copy(), 4, 4, cap=False))
+    assert dA.info.block_slots == 12
+    x = matrices.uniform_vector(A.width, seed=41)
+    y0 = matrices.uniform_vector(A.height, seed=42)
+    dy = cs.DeviceVector(ctx, A.height, y0)
+    dA.spmv(cs.DeviceVector(ctx, A.width, x), dy)
+    ctx.sync()
+    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+    err, worst = oracle.scaled_error(dy.download(), y64, bound)
+    assert err <= TOL, f"scaled error {err:.3e} at row {worst}"
 
 
 @pytest.mark.parametrize("dtype", [cs.DT_BF16, cs.DT_F16])

@@ -143,12 +143,49 @@ def test_bdia(ctx, oracle, name, method):
 BLOCK_CASES = ["block_small", "rand_small", "rand_dyadic", "lap2d_256", "tall", "wide"]
 
 
+def _bcsr_slots(b) -> int:
+    """Slots per block row of the slot-major BCSR encoding the upload picks
+    (every block row at most 64 blocks, padding to the longest at most 10 %
+    of the blocks), or 0 for the row-pointer encoding."""
+    h, rp = b.header(), b.arrays()["rp"]
+    nbr, nb = h["b4csr_row_num"], h["b4csr_block_num"]
+    w = int(np.diff(rp).max()) if nbr else 0
+    return w if nb > 0 and w <= 64 and w * nbr * 10 <= nb * 11 else 0
+
+
 @pytest.mark.parametrize("name", BLOCK_CASES)
 @pytest.mark.parametrize("shape", cs.BLOCK_SHAPES)
 def test_bcsr(ctx, oracle, name, shape):
+    """Both BCSR encodings (block_small and most lap2d shapes slot-major,
+    the random matrices row-pointer) against the oracle."""
     A = CASES[name]
     bh, bw = shape
-    _run(ctx, oracle, A, cs.coo2b4csr(A.copy(), bw, bh), 0)
+    b = cs.coo2b4csr(A.copy(), bw, bh)
+    assert cs.DeviceMatrix(ctx, b).info.block_slots == _bcsr_slots(b)
+    _run(ctx, oracle, A, b, 0)
+
+
+def _ragged_blocks(bh, bw):
+    """Dyadic block rows of 6 blocks, every 20th block row missing one (a
+    padding slot in the slot-major encoding)."""
+    A = matrices.block_stencil(900, bh, bw, seed=13, offsets=(0, 1, -1, 32, -33, 5), dyadic=True)
+    brow, bcol = A.row // bh, A.col // bw
+    drop = (brow % 20 == 7) & (bcol == (brow + 5) % 900)
+    return cs.Coo(A.row[~drop], A.col[~drop], A.val[~drop], A.height, A.width)
+
+
+@pytest.mark.parametrize("shape", [(4, 4), (2, 8), (8, 4), (1, 4)])
+@pytest.mark.parametrize("x_offset", [0, 1])
+def test_bcsr_slot_major_ragged(ctx, oracle, shape, x_offset):
+    """Slot-major BCSR with padding slots: bit for bit the oracle's sum on
+    dyadic data, with x aligned and unaligned for float4 loads."""
+    bh, bw = shape
+    A = _ragged_blocks(bh, bw)
+    b = cs.coo2b4csr(A.copy(), bw, bh)
+    hdr = b.header()
+    assert 0 < _bcsr_slots(b) and _bcsr_slots(b) * hdr["b4csr_row_num"] > hdr["b4csr_block_num"]
+    assert cs.DeviceMatrix(ctx, b).info.block_slots == _bcsr_slots(b)
+    _run(ctx, oracle, A, b, 0, x_offset=x_offset)
 
 
 @pytest.mark.parametrize("name", BLOCK_CASES)
