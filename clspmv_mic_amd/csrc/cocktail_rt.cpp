@@ -42,6 +42,11 @@ struct clspmv_cocktail_kernels {
     clspmv_cocktail_gpu* gpu = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // kGraphPasses passes in one graph: do_spmv(ntimes) launches it while
+    // ntimes allows, so a pass of a small matrix is not held to the host's
+    // launch rate.
+    hipGraph_t graph_many = nullptr;
+    hipGraphExec_t exec_many = nullptr;
     bool use_graph = true;  // false: the pass is issued as plain launches
     // Panel first passes on `side`, forked from and joined back into the
     // launching stream by the two events; false: one stream throughout.
@@ -136,14 +141,14 @@ int launch_all(hipStream_t s, const clspmv_cocktail_kernels* k) {
     return CLSPMV_OK;
 }
 
-int capture(clspmv_cocktail_kernels* k) {
+constexpr int kGraphPasses = 16;
+
+// Captures `passes` consecutive passes into *graph / *exec.
+int capture_passes(clspmv_cocktail_kernels* k, int passes, hipGraph_t* graph, hipGraphExec_t* exec) {
     hipStream_t s = k->ctx->stream;
-    if (k->exec) (void)hipGraphExecDestroy(k->exec);
-    if (k->graph) (void)hipGraphDestroy(k->graph);
-    k->exec = nullptr;
-    k->graph = nullptr;
     CLSPMV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int st = launch_all(s, k);
+    int st = CLSPMV_OK;
+    for (int i = 0; i < passes && !st; ++i) st = launch_all(s, k);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (st) {
@@ -151,9 +156,28 @@ int capture(clspmv_cocktail_kernels* k) {
         return st;
     }
     if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    k->graph = g;
-    CLSPMV_HIP(hipGraphInstantiate(&k->exec, g, nullptr, nullptr, 0));
+    *graph = g;
+    CLSPMV_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
     return CLSPMV_OK;
+}
+
+void drop_graphs(clspmv_cocktail_kernels* k) {
+    if (k->exec) (void)hipGraphExecDestroy(k->exec);
+    if (k->graph) (void)hipGraphDestroy(k->graph);
+    if (k->exec_many) (void)hipGraphExecDestroy(k->exec_many);
+    if (k->graph_many) (void)hipGraphDestroy(k->graph_many);
+    k->exec = k->exec_many = nullptr;
+    k->graph = k->graph_many = nullptr;
+}
+
+// One pass (exec) and kGraphPasses passes (exec_many) as graphs.  On an
+// error neither is left, and do_spmv refuses to run.
+int capture(clspmv_cocktail_kernels* k) {
+    drop_graphs(k);
+    int st = capture_passes(k, 1, &k->graph, &k->exec);
+    if (!st) st = capture_passes(k, kGraphPasses, &k->graph_many, &k->exec_many);
+    if (st) drop_graphs(k);
+    return st;
 }
 
 template <class F>
@@ -186,8 +210,7 @@ void destroy_gpu(clspmv_cocktail_gpu* g) {
 
 void destroy_kernels(clspmv_cocktail_kernels* k) {
     if (!k) return;
-    if (k->exec) (void)hipGraphExecDestroy(k->exec);
-    if (k->graph) (void)hipGraphDestroy(k->graph);
+    drop_graphs(k);
     if (k->fork) (void)hipEventDestroy(k->fork);
     if (k->join) (void)hipEventDestroy(k->join);
     if (k->side) (void)hipStreamDestroy(k->side);
@@ -296,7 +319,9 @@ int clspmv_do_spmv_async(clspmv_cocktail_kernels* k, int ntimes) {
             if ((st = launch_all(k->ctx->stream, k))) return st;
         return CLSPMV_OK;
     }
-    for (int i = 0; i < ntimes; ++i) CLSPMV_HIP(hipGraphLaunch(k->exec, k->ctx->stream));
+    int i = 0;
+    for (; i + kGraphPasses <= ntimes; i += kGraphPasses) CLSPMV_HIP(hipGraphLaunch(k->exec_many, k->ctx->stream));
+    for (; i < ntimes; ++i) CLSPMV_HIP(hipGraphLaunch(k->exec, k->ctx->stream));
     return CLSPMV_OK;
 }
 
@@ -602,13 +627,14 @@ int clspmv_cocktail_tune(clspmv_cocktail_kernels* k, int ntimes, int* graph) {
     // everywhere: graph nodes of one stream are spaced further apart than
     // plain launches, and a second stream costs plain launches an event
     // round trip per pass.
-    auto best_of3 = [&](auto&& fn, double* out) -> int {
+    // runs calls of fn per batch, each `per` passes: ms per pass.
+    auto best_of3 = [&](auto&& fn, int runs, int per, double* out) -> int {
         double best = 1e30;
         for (int rep = 0; rep < 3; ++rep) {
             double t = 0.0;
-            int r = time_runs(k, ntimes, fn, &t);
+            int r = time_runs(k, runs, fn, &t);
             if (r) return r;
-            best = std::min(best, t);
+            best = std::min(best, t / per);
         }
         *out = best;
         return CLSPMV_OK;
@@ -623,12 +649,13 @@ int clspmv_cocktail_tune(clspmv_cocktail_kernels* k, int ntimes, int* graph) {
         k->overlap = modes[m].first;
         k->fuse = modes[m].second;
         double t_graph = 0.0, t_direct = 0.0;
+        // Graph launches as do_spmv issues them (kGraphPasses passes each).
         if (!(st = capture(k)))
             st = best_of3([&](hipStream_t q) -> int {
-                CLSPMV_HIP(hipGraphLaunch(k->exec, q));
+                CLSPMV_HIP(hipGraphLaunch(k->exec_many, q));
                 return CLSPMV_OK;
-            }, &t_graph);
-        if (!st) st = best_of3([&](hipStream_t q) { return launch_all(q, k); }, &t_direct);
+            }, std::max(2, ntimes / kGraphPasses), kGraphPasses, &t_graph);
+        if (!st) st = best_of3([&](hipStream_t q) { return launch_all(q, k); }, ntimes, 1, &t_direct);
         if (st) break;
         if (t_graph < best_ms) best_ms = t_graph, best_graph = true, best_overlap = k->overlap, best_fuse = k->fuse;
         if (t_direct < best_ms) best_ms = t_direct, best_graph = false, best_overlap = k->overlap, best_fuse = k->fuse;

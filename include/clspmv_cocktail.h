@@ -123,7 +123,8 @@ int clspmv_init_mat_kernels_on(int device_ordinal, const clspmv_cocktail* cpumat
 
 /* do_spmv (src/spmv_cocktail.cpp:1088): ntimes passes of y += A*x over all
  * parts in the reference's order BDIA, DIA, SBELL, BELL, BCSR, SELL, ELL,
- * CSR, COO; blocks until done. */
+ * CSR, COO; blocks until done.  In graph mode passes go out 16 to a graph
+ * launch while ntimes allows (the same kernels in the same order). */
 int clspmv_do_spmv(clspmv_cocktail_kernels* kernels, int ntimes);
 /* Same without the final wait (work stays queued on the kernels' stream). */
 int clspmv_do_spmv_async(clspmv_cocktail_kernels* kernels, int ntimes);
@@ -177,8 +178,8 @@ int clspmv_cocktail_time_cold(clspmv_cocktail_kernels* kernels, int ntimes, size
  * parts
  * are uploaded in both their own and the x-panel encoding, and the losing
  * encoding is freed); then the captured
- * graph is re-built, and graph launches and plain stream launches of the
- * same pass are timed, each with the x-panel first passes on a second
+ * graphs are re-built, and graph launches (as clspmv_do_spmv issues them)
+ * and plain stream launches of the same pass are timed, each with the x-panel first passes on a second
  * stream (clspmv_cocktail_set_overlap), on one stream, and with a DIA part
  * fused into a first-pass launch (clspmv_cocktail_set_fuse); the fastest is
  * kept.  y is preserved.  Tuning picks

@@ -213,6 +213,38 @@ def test_dia_in_first_pass_launch(oracle):
     np.testing.assert_array_equal(got, y64.astype(np.float32), err_msg=str(modes))
 
 
+def test_many_passes_graph_equals_direct(oracle):
+    """do_spmv(ntimes) in graph mode launches a graph of 16 passes while
+    ntimes allows, then single passes: 37 passes give bit for bit the y of
+    37 plain launches, in every launch mode (x-panel first pass on a second
+    stream or not, the DIA part fused into the first pass or not), and stay
+    within the tolerance of y0 + 37 A x."""
+    A = matrices.power_law(500000, seed=7)  # DIA (the band) + COO under the shipped tables
+    ck = cs.analyze_matrix(A.copy(), cs.TABLES_DIR)
+    x = matrices.uniform_vector(A.width, seed=51)
+    y0 = matrices.uniform_vector(A.height, seed=52)
+    rt = cs.CocktailRuntime(ck, x, y0)
+    n = 37
+    outs = {}
+    for overlap, fuse in ((True, False), (False, False), (False, True)):
+        rt.set_overlap(overlap)
+        rt.set_fuse(fuse)
+        for graph in (True, False):
+            rt.set_graph(graph)
+            rt.set_y(y0)
+            rt.do_spmv(n)
+            outs[(overlap, fuse, graph)] = rt.get_y()
+    rt.close()
+    for overlap, fuse in ((True, False), (False, False), (False, True)):
+        np.testing.assert_array_equal(outs[(overlap, fuse, True)], outs[(overlap, fuse, False)],
+                                      err_msg=f"overlap={overlap} fuse={fuse}")
+    y64, bound = oracle.spmv_f64(A.row, A.col, A.val, x, A.height, y0)
+    want = y0.astype(np.float64) + n * (y64 - y0.astype(np.float64))
+    for key, y in outs.items():
+        err, worst = oracle.scaled_error(y, want, n * bound)
+        assert err <= n * TOL, f"{key}: scaled error {err:.3e} at row {worst}"
+
+
 def test_failed_tune_leaves_runtime_usable(oracle):
     """A tuning call that fails part-way (a host-side error injected at trial
     n, for every n up to the last trial) returns the error, leaves y as it
