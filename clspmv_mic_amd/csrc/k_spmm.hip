@@ -243,9 +243,17 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
             if (xc >= k) continue;
             float* yp = Y + (size_t)xc * ldy + r0;
             if (r0 + 3 < rows) {
-                f32x4 y = *reinterpret_cast<const f32x4*>(yp);
+                // Several passes over the slice (tiled, more than 4 * NQ
+                // right-hand sides): Y is read and written once per pass,
+                // with non-temporal accesses, so it does not evict the
+                // slice's blocks before the next pass re-reads them.
+                f32x4* yq = reinterpret_cast<f32x4*>(yp);
+                f32x4 y;
+                if constexpr (TILED && !NT) y = __builtin_nontemporal_load(yq);
+                else y = *yq;
                 y += acc[q];
-                *reinterpret_cast<f32x4*>(yp) = y;
+                if constexpr (TILED && !NT) __builtin_nontemporal_store(y, yq);
+                else *yq = y;
             } else {
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
@@ -294,7 +302,10 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
     // One pass over A (k <= 4 * NQ): non-temporal A loads.  Measured on the
     // 524,288 x 12 4x4-block matrix: k = 8 / 16 in 76.6 / 122 us against
     // 78.5 / 130 us with normal loads; at k = 32 (two passes) normal loads
-    // keep A in cache for the second pass (247 us against 262 us).
+    // keep A in cache for the second pass (247 us against 262 us), and
+    // non-temporal Y accesses keep more of it there (236.5 us, 1.30 x the
+    // algorithmic bytes against 1.34 x; the same Y accesses cost 3-5 us at
+    // k = 8 / 16).
     const bool nt = k <= 4 * NQ;
     const unsigned nb = (unsigned)blocks;
     if (long_rows) {
