@@ -90,27 +90,28 @@ __global__ __launch_bounds__(kBlock) void dia_multi_kernel(
 }
 
 // Accumulates a[q] += val[j] * X[col[j]][qb + q] over j = j0, j0 + step, ...
-// below j1, four nonzeros in flight per lane.  Acc is float, or double for
-// rows longer than kLongRowF64 (the products are exact in double).
-template <int Q, class V, class Acc>
+// below j1 in that order, U nonzeros in flight per lane.  Acc is float, or
+// double for rows longer than kLongRowF64 (the products are exact in
+// double).
+template <int Q, class V, class Acc, int U>
 __device__ __forceinline__ void csr_multi_run(const int* __restrict__ col,
                                               const typename V::T* __restrict__ val, int j0, int j1,
                                               int step, const float* __restrict__ X, int k, int qb,
                                               bool vec, Acc* a) {
     int j = j0;
-    for (; j + 3 * step < j1; j += 4 * step) {
-        int c[4];
-        float v[4];
+    for (; j + (U - 1) * step < j1; j += U * step) {
+        int c[U];
+        float v[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             c[u] = ld_stream(col + j + u * step);
             v[u] = V::ld(val + j + u * step);
         }
-        float xv[4][Q];
+        float xv[U][Q];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) load_xq<Q>(X, c[u], k, qb, vec, xv[u]);
+        for (int u = 0; u < U; ++u) load_xq<Q>(X, c[u], k, qb, vec, xv[u]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int q = 0; q < Q; ++q) a[q] = fma((Acc)v[u], (Acc)xv[u][q], a[q]);
     }
@@ -125,8 +126,9 @@ __device__ __forceinline__ void csr_multi_run(const int* __restrict__ col,
 }
 
 // One row longer than kAdaptiveNnz by the whole work-group: each lane
-// strides the row, the four waves' sums are combined in a fixed order
-// through LDS, and Y[:, r] is updated once per right-hand side.
+// strides the row with 4 * L nonzeros in flight (a whole work-group round
+// covers kAdaptiveNnz of them), the four waves' sums are combined in a
+// fixed order through LDS, and Y[:, r] is updated once per right-hand side.
 template <int KB, class V, class Acc>
 __device__ __forceinline__ void csr_multi_long(const typename V::T* __restrict__ val,
                                                const int* __restrict__ col, int r, int j0, int j1,
@@ -139,7 +141,8 @@ __device__ __forceinline__ void csr_multi_long(const typename V::T* __restrict__
     Acc a[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) a[q] = 0;
-    csr_multi_run<Q, V, Acc>(col, val, j0 + t / L, j1, kBlock / L, X, k, qb, vec, a);
+    csr_multi_run<Q, V, Acc, launch::kAdaptiveNnz * L / kBlock>(col, val, j0 + t / L, j1, kBlock / L, X, k, qb,
+                                                                vec, a);
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         for (int m = kWave >> 1; m >= L; m >>= 1) a[q] += __shfl_xor(a[q], m, kWave);
@@ -160,16 +163,26 @@ __device__ __forceinline__ void csr_multi_long(const typename V::T* __restrict__
     }
 }
 
-// CSR over the adaptive row blocks (runs of rows whose nonzeros fit
-// kAdaptiveNnz, or one longer row).  Each lane carries Q = min(KB, 4)
-// right-hand sides (one 16-byte gather of X per nonzero), and L = KB / Q
-// neighbouring lanes cover the k values of one nonzero.  Short rows: a
-// group of G lanes per row, G the largest power of two with the block's
-// rows fitting the work-group (at least L, at most a wave; rows beyond the
-// work-group loop).  A long row: the whole work-group, the four waves'
-// partial sums combined in a fixed order through LDS, in FP64 for rows
-// longer than kLongRowF64 (products exact, one rounding into y).  No
-// product staging, so occupancy is set by registers alone.
+// CSR over the adaptive row blocks (runs of at most kAdaptiveRows rows whose
+// nonzeros fit kAdaptiveNnz, or one longer row).  Each lane carries
+// Q = min(KB, 4) right-hand sides (one 16-byte gather of X per nonzero), and
+// L = KB / Q neighbouring lanes cover the k values of one nonzero.
+//
+// A block of short rows is streamed: its nonzeros are cut into G = kBlock / L
+// equal consecutive chunks, one per lane group, whatever the rows' lengths
+// (a row-per-group split left short rows' groups idle beside one long row's
+// and put the gathers of a group's rows one after another).  A group loads
+// every column, value and X row of its chunk at once, then walks the chunk
+// in order: a row that starts and ends inside it is summed there and added
+// to Y; the part of a row that began in an earlier chunk goes to LDS (head),
+// and a row that continues past the chunk is finished, after a barrier, by
+// the group where it starts (its tail, then the later chunks' heads in
+// order).  Each row is thus summed in nonzero order with one break per chunk
+// boundary, in FP32, the same way every time.
+//
+// A long row: the whole work-group, the four waves' partial sums combined in
+// a fixed order through LDS, in FP64 for rows longer than kLongRowF64
+// (products exact, one rounding into y).
 template <int KB, class V>
 __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
     const int* __restrict__ blocks, const int* __restrict__ rp, const int* __restrict__ col,
@@ -177,15 +190,18 @@ __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
     float* __restrict__ Y, int ldy) {
     constexpr int Q = KB < 4 ? KB : 4;
     constexpr int L = KB / Q;
+    constexpr int G = kBlock / L;                   // lane groups
+    constexpr int CM = launch::kAdaptiveNnz / G;    // nonzeros per group at most
+    static_assert(CM * G == launch::kAdaptiveNnz, "a block's nonzeros must split evenly over the groups");
     __shared__ float part[kBlock / kWave][KB];
     __shared__ double part64[kBlock / kWave][KB];
+    __shared__ int srp[launch::kAdaptiveRows + 1];  // the block's row starts, from its first nonzero
+    __shared__ float head[G][KB];                   // each chunk's part of a row begun earlier
     const int b = blockIdx.x;
     const int r0 = blocks[b], r1 = blocks[b + 1];
     const int nrows = r1 - r0;
     const int t = threadIdx.x;
-    const int sub = t % L;
-    const int qb = sub * Q;
-    float a[Q];
+    const int qb = (t % L) * Q;
     if (nrows == 1 && rp[r1] - rp[r0] > launch::kAdaptiveNnz) {
         if (rp[r1] - rp[r0] > launch::kLongRowF64)
             csr_multi_long<KB, V, double>(val, col, r0, rp[r0], rp[r1], X, k, vec, Y, ldy, part64);
@@ -193,21 +209,87 @@ __global__ __launch_bounds__(kBlock) void csr_multi_kernel(
             csr_multi_long<KB, V, float>(val, col, r0, rp[r0], rp[r1], X, k, vec, Y, ldy, part);
         return;
     }
-    int g = kBlock / (nrows > 0 ? nrows : 1);
-    g = g >= 64 ? 64 : g >= 32 ? 32 : g >= 16 ? 16 : g >= 8 ? 8 : g >= 4 ? 4 : g >= 2 ? 2 : 1;
-    if (g < L) g = L;
-    const int lane = t & (g - 1);
-    for (int r = t / g; r < nrows; r += kBlock / g) {  // uniform per group
+    const int s = rp[r0];
+    const int n = rp[r1] - s;  // at most kAdaptiveNnz
+    for (int i = t; i <= nrows; i += kBlock) srp[i] = rp[r0 + i] - s;
+    __syncthreads();
+    if (n == 0) return;  // only empty rows (the whole work-group)
+    const int gi = t / L;
+    const int C = (n + G - 1) / G;  // <= CM
+    const int c0 = min(n, gi * C), c1 = min(n, c0 + C);
+    // Every load of the chunk before its first product (slots past the
+    // chunk read nonzero s and are never used).
+    int cc[CM];
+    float vv[CM], xv[CM][Q];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) a[q] = 0.0f;
-        csr_multi_run<Q, V, float>(col, val, rp[r0 + r] + lane / L, rp[r0 + r + 1], g / L, X, k, qb, vec, a);
+    for (int u = 0; u < CM; ++u) {
+        const int j = c0 + u < c1 ? s + c0 + u : s;
+        cc[u] = ld_stream(col + j);
+        vv[u] = V::ld(val + j);
+    }
 #pragma unroll
-        for (int q = 0; q < Q; ++q) a[q] = xor_sum(a[q], g, L);
-        if (lane < L) {
+    for (int u = 0; u < CM; ++u) load_xq<Q>(X, cc[u], k, qb, vec, xv[u]);
+    float acc[Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (qb + q < k) Y[(size_t)(qb + q) * ldy + r0 + r] += a[q];
+    for (int q = 0; q < Q; ++q) acc[q] = 0.0f;
+    auto add_y = [&](int r) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (qb + q < k) Y[(size_t)(qb + q) * ldy + r0 + r] += acc[q];
+    };
+    int r = 0, rend = 0;
+    bool in_head = false;  // the current row began before this chunk
+    if (c0 < c1) {
+        // The row holding nonzero c0: the last row starting at or before it
+        // (empty rows before it start at the same place and are passed).
+        int lo = 0, hi = nrows - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (srp[mid] <= c0) lo = mid;
+            else hi = mid - 1;
         }
+        r = lo;
+        rend = srp[r + 1];
+        in_head = srp[r] < c0;
+#pragma unroll
+        for (int u = 0; u < CM; ++u) {
+            const int j = c0 + u;
+            if (j < c1) {
+                if (j >= rend) {  // row r ended inside the chunk
+                    if (in_head) {
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) head[gi][qb + q] = acc[q];
+                    } else {
+                        add_y(r);
+                    }
+                    in_head = false;
+                    do {
+                        ++r;
+                        rend = srp[r + 1];
+                    } while (j >= rend);  // empty rows
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) acc[q] = 0.0f;
+                }
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc[q] = fmaf(vv[u], xv[u][q], acc[q]);
+            }
+        }
+        if (in_head) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) head[gi][qb + q] = acc[q];
+        } else if (rend <= c1) {
+            add_y(r);
+        }
+    }
+    __syncthreads();
+    // The row this chunk ends inside of and began: its tail, then the heads
+    // of the later chunks it runs through, in order.
+    if (c0 < c1 && !in_head && rend > c1) {
+        const int last = (rend - 1) / C;  // the chunk holding the row's last nonzero
+        for (int g2 = gi + 1; g2 <= last; ++g2)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[q] += head[g2][qb + q];
+        add_y(r);
     }
 }
 

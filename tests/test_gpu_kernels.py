@@ -26,6 +26,10 @@ def _cases():
                                                  long_len=2500),
         "rand_long_dyadic": matrices.random_sparse(1500, 1500, 0.003, seed=5, dyadic=True,
                                                    long_rows=3, long_len=1400),
+        # rows of up to ~1000 entries inside the short-row blocks (spanning
+        # many stream chunks), empty rows, dyadic values
+        "rand_mid_dyadic": matrices.random_sparse(3000, 2000, 0.002, seed=14, dyadic=True,
+                                                  empty_rows=0.2, long_rows=6, long_len=1000),
         "lap2d_256": matrices.laplacian_2d(256),
         "stencil27_16": matrices.stencil_27pt(16, seed=3),
         "stencil27_dyadic": matrices.stencil_27pt(12, seed=5, dyadic=True),
@@ -337,7 +341,7 @@ def test_16bit_needs_panel(ctx):
             cs.DeviceMatrix(ctx, A.sorted(), panel=panel, dtype=cs.DT_BF16)
 
 
-MULTI_CASES = ["rand_small", "rand_dyadic", "rand_long_rows", "rand_long_dyadic", "lap2d_256",
+MULTI_CASES = ["rand_small", "rand_dyadic", "rand_long_rows", "rand_long_dyadic", "rand_mid_dyadic", "lap2d_256",
                "stencil27_dyadic", "banded", "tall", "wide"]
 
 
