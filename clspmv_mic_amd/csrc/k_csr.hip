@@ -56,11 +56,26 @@ __global__ __launch_bounds__(kBlock) void csr_adaptive_kernel(
     const int nrows = r1 - r0;
     const int t = threadIdx.x;
 
+    constexpr int U = launch::kAdaptiveNnz / kBlock;  // nonzeros in flight per thread
     if (nrows == 1 && e - s > launch::kAdaptiveNnz) {
-        // Long row: the whole work-group strides over it.
+        // Long row: the whole work-group strides over it, U nonzeros' loads
+        // in flight per thread (each thread's sum still in index order).
         float acc = 0.0f;
-        for (int k = s + t; k < e; k += kBlock)
-            acc = fmaf(ld_stream(val + k), x[ld_stream(col + k)], acc);
+        int k = s + t;
+        for (; k + (U - 1) * kBlock < e; k += U * kBlock) {
+            int c[U];
+            float v[U], xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c[u] = ld_stream(col + k + u * kBlock);
+                v[u] = ld_stream(val + k + u * kBlock);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc = fmaf(v[u], xv[u], acc);
+        }
+        for (; k < e; k += kBlock) acc = fmaf(ld_stream(val + k), x[ld_stream(col + k)], acc);
         acc = group_sum<kWave>(acc);
         if ((t & (kWave - 1)) == 0) wave_part[t / kWave] = acc;
         __syncthreads();
@@ -73,9 +88,25 @@ __global__ __launch_bounds__(kBlock) void csr_adaptive_kernel(
         return;
     }
 
-    // Stage products with coalesced loads.
-    for (int k = t; k < e - s; k += kBlock)
-        prod[k] = ld_stream(val + s + k) * x[ld_stream(col + s + k)];
+    // Stage products with coalesced loads, every load of the block in
+    // flight at once (e - s <= kAdaptiveNnz = U * kBlock).  A block of empty
+    // rows loads nothing (its s may be the matrix's last index + 1).
+    if (e > s) {
+        int c[U];
+        float v[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = t + u * kBlock;
+            const int kk = k < e - s ? s + k : s;  // past the block: a valid index, not stored
+            c[u] = ld_stream(col + kk);
+            v[u] = ld_stream(val + kk);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (t + u * kBlock < e - s) prod[t + u * kBlock] = v[u] * xv[u];
+    }
     __syncthreads();
 
     // Lane-group size: the largest power of two with nrows groups fitting
