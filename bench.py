@@ -250,10 +250,11 @@ def measure_cocktail(args, name):
     }
 
 
-def spmm_bytes(blocks, brows, rows, cols, k, s_v=2, s_x=2):
-    """Algorithmic bytes of Y += A X with 4x4 blocks: dense block values,
-    block column ids and block row pointers, X once, Y read and written."""
-    return blocks * 16 * s_v + 4 * blocks + 4 * (brows + 1) + s_x * cols * k + 2 * 4 * rows * k
+def spmm_bytes(blocks, brows, rows, cols, k, s_v=2, s_x=2, y_read=True):
+    """Algorithmic bytes of Y = A X (y_read False) or Y += A X with 4x4
+    blocks: dense block values, block column ids and block row pointers, X
+    once, Y written (and read)."""
+    return blocks * 16 * s_v + 4 * blocks + 4 * (brows + 1) + s_x * cols * k + (2 if y_read else 1) * 4 * rows * k
 
 
 def run_spmm(args):
@@ -290,12 +291,13 @@ def run_spmm(args):
         if tiled:  # X handed over in 4-row tiles: packed once, outside the timed region
             dX4 = cs.spmm_pack_x(ctx, k, A.width, dX, ldx)
 
-        def product():
+        def product(accumulate=False):
             if tiled:
-                M.spmm_tiled(k, dX4, dY, ldy)
+                M.spmm_tiled(k, dX4, dY, ldy, accumulate=accumulate)
             else:
-                M.spmm(k, dX, ldx, dY, ldy)
+                M.spmm(k, dX, ldx, dY, ldy)  # Y += A X only
 
+        overwrite = tiled  # Y = A X, as the workload defines it
         product()
         ctx.sync()
         # Error per right-hand side against the FP64 SpMV on the GPU: of the
@@ -314,6 +316,16 @@ def run_spmm(args):
             product()
         ctx.sync()
         ms = (time.perf_counter() - t1) * 1e3 / args.steps
+        # Y += A X (Y read as well), timed the same way.
+        acc_ms = None
+        if tiled and not args.no_accumulate:
+            product(accumulate=True)
+            ctx.sync()
+            t2 = time.perf_counter()
+            for _ in range(args.steps):
+                product(accumulate=True)
+            ctx.sync()
+            acc_ms = (time.perf_counter() - t2) * 1e3 / args.steps
         # The other X layout, timed the same way, for comparison.
         other_ms = None
         if tiled:
@@ -325,7 +337,8 @@ def run_spmm(args):
             ctx.sync()
             other_ms = (time.perf_counter() - t2) * 1e3 / args.steps
         flops = 2.0 * A.nnz * k
-        nbytes = spmm_bytes(blocks, brows, A.height, A.width, k)
+        nbytes = spmm_bytes(blocks, brows, A.height, A.width, k, y_read=not overwrite)
+        acc_bytes = spmm_bytes(blocks, brows, A.height, A.width, k)
         out = {
             "metric": "block SpMM GFLOP/s (BF16/FP16 values, FP32 accumulation on MFMA)",
             "value": round(flops / (ms * 1e-3) / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1,
@@ -334,13 +347,19 @@ def run_spmm(args):
             "data": "synthetic",
             "config": {"workload": "4x4-block matrix, 524,288 block rows x 12 blocks, X uniform[-1,1]",
                        "rows": A.height, "nnz": A.nnz, "blocks": blocks, "k": k,
-                       "x_layout": "4-row tiles" if tiled else "column-major"},
+                       "x_layout": "4-row tiles" if tiled else "column-major",
+                       "product": "Y = A X" if overwrite else "Y += A X"},
             "roofline": {"bound": "hbm", "achieved": round(nbytes / (ms * 1e-3) / 1e9, 2), "peak": PEAK_HBM_GBPS,
                          "unit": "GB/s", "frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
                          # counters of the BF16 tiled kernel at this k, where measured
-                         "traffic": measured_traffic(f"spmm-k{k}", "bspmm4_kernel")
+                         "traffic": measured_traffic(f"spmm-set-k{k}" if overwrite else f"spmm-k{k}",
+                                                     "bspmm4_kernel")
                          if tiled and args.dtype == "bf16" else None,
                          "algorithmic_bytes": nbytes},
+            "y_accumulate": None if acc_ms is None else {
+                "ms": round(acc_ms, 6), "algorithmic_bytes": acc_bytes,
+                "frac": round(acc_bytes / (acc_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
+                "traffic": measured_traffic(f"spmm-k{k}", "bspmm4_kernel") if args.dtype == "bf16" else None},
             "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
             "column_major_x_ms": round(other_ms, 6) if other_ms is not None else None,
             "error_vs_fp64_stored_values": [float(f"{e:.3e}") for e in errs],
@@ -644,6 +663,8 @@ def main():
     ap.add_argument("--dtype", default="bf16", choices=["bf16", "fp16"], help="spmm: value type")
     ap.add_argument("--k", type=int, nargs="+", default=[8, 16, 32], help="spmm: right-hand sides")
     ap.add_argument("--err-columns", type=int, default=4, help="spmm: columns checked against FP64")
+    ap.add_argument("--no-accumulate", action="store_true",
+                    help="spmm: time only Y = A X, not also Y += A X")
     ap.add_argument("--x-layout", default="tiled", choices=["tiled", "column"],
                     help="spmm: X in 4-row tiles (clspmv_spmm_tiled) or column-major (clspmv_spmm)")
     ap.add_argument("--matrices", type=int, default=64, help="serve: resident matrices")

@@ -264,6 +264,7 @@ _sig("clspmv_verify_spmv_noalloc", C.c_int, VP, VP, VP, VP, VP, C.c_double, VP)
 NOALLOC_MAX_ROW_BLOCKS = 131072
 _sig("clspmv_spmm", C.c_int, VP, VP, C.c_int, VP, C.c_int, VP, C.c_int)
 _sig("clspmv_spmm_tiled", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int)
+_sig("clspmv_spmm_tiled_ex", C.c_int, VP, VP, C.c_int, VP, VP, C.c_int, C.c_int)
 _sig("clspmv_spmm_pack_x", C.c_int, VP, C.c_int, C.c_int, VP, C.c_int, VP)
 _sig("clspmv_convert_from_f32", C.c_int, PF, VP, C.c_size_t, C.c_int)
 _sig("clspmv_convert_to_f32", C.c_int, VP, PF, C.c_size_t, C.c_int)
@@ -810,14 +811,15 @@ class BlockSpmm:
         self._check_y(k, Y, ldy)
         check(lib.clspmv_spmm(self.ctx.h, self.h, k, X.p, ldx, Y.p, ldy), "spmm")
 
-    def spmm_tiled(self, k: int, X4, Y, ldy: int):
-        """Y += A X with X in 4-row tiles (clspmv_spmm_tiled; spmm_pack_x
-        builds them from a column-major X)."""
+    def spmm_tiled(self, k: int, X4, Y, ldy: int, accumulate: bool = True):
+        """Y += A X (accumulate) or Y = A X, Y not read, with X in 4-row
+        tiles (clspmv_spmm_tiled_ex; spmm_pack_x builds them from a
+        column-major X)."""
         need = 8 * ((self.info.cols + 3) // 4) * k
         if _nbytes(X4) < need:
             raise ClspmvError(ERR_INVALID, "spmm_tiled", f"X4 holds {_nbytes(X4)} bytes, k={k} need {need}")
         self._check_y(k, Y, ldy)
-        check(lib.clspmv_spmm_tiled(self.ctx.h, self.h, k, X4.p, Y.p, ldy), "spmm_tiled")
+        check(lib.clspmv_spmm_tiled_ex(self.ctx.h, self.h, k, X4.p, Y.p, ldy, int(accumulate)), "spmm_tiled")
 
     def error_report(self, ref: "DeviceMatrix", seed: int = 1, tol: float = 1e-5) -> dict:
         """Error of one SpMM column (x rounded to the matrix's type) against

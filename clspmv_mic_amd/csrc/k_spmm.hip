@@ -79,7 +79,7 @@ template <bool BF16, int NQ, bool LONG, bool TILED, bool X16, bool NT>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int brows, int rows, int cols, int kall, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
-    const unsigned short* __restrict__ Xall, int ldx, float* __restrict__ Yall, int ldy) {
+    const unsigned short* __restrict__ Xall, int ldx, float* __restrict__ Yall, int ldy, bool acc_y) {
     static_assert(!X16 || (TILED && NQ >= 2), "16-byte loads take two tiles of the tiled layout");
     static_assert(!NT || TILED, "non-temporal A loads belong to the tiled kernel");
     static_assert(launch::kSpmmSlice * 4 == kWave, "a wave holds one slice, four lanes per block row");
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
                 float* yp = Y + (size_t)xc * ldy + r0;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (r0 + i < rows) yp[i] = (float)((double)yp[i] + acc64[q][i]);
+                    if (r0 + i < rows) yp[i] = (float)((acc_y ? (double)yp[i] : 0.0) + acc64[q][i]);
             }
             return;
         }
@@ -247,17 +247,21 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
                 // right-hand sides): Y is read and written once per pass,
                 // with non-temporal accesses, so it does not evict the
                 // slice's blocks before the next pass re-reads them.
+                // Y += A X, or Y = A X with Y not read (then 0 + the sum, the
+                // bits a zeroed Y would give).
                 f32x4* yq = reinterpret_cast<f32x4*>(yp);
-                f32x4 y;
-                if constexpr (TILED && !NT) y = __builtin_nontemporal_load(yq);
-                else y = *yq;
+                f32x4 y = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                if (acc_y) {
+                    if constexpr (TILED && !NT) y = __builtin_nontemporal_load(yq);
+                    else y = *yq;
+                }
                 y += acc[q];
                 if constexpr (TILED && !NT) __builtin_nontemporal_store(y, yq);
                 else *yq = y;
             } else {
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (r0 + i < rows) yp[i] += acc[q][i];
+                    if (r0 + i < rows) yp[i] = (acc_y ? yp[i] : 0.0f) + acc[q][i];
             }
         }
     };
@@ -278,19 +282,20 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
 
 template <bool BF16, int NQ, bool LONG, bool TILED, bool X16>
 void launch_kernel(hipStream_t s, unsigned blocks, bool nt, int brows, int rows, int cols, int k, const int* rp,
-                   const int* bcol, const unsigned short* val, const unsigned short* X, int ldx, float* Y, int ldy) {
+                   const int* bcol, const unsigned short* val, const unsigned short* X, int ldx, float* Y, int ldy,
+                   bool acc_y) {
     if (TILED && nt)
         hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, TILED>), dim3(blocks), dim3(kBlock), 0, s,
-                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y);
     else
         hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, false>), dim3(blocks), dim3(kBlock), 0, s,
-                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y);
 }
 
 template <bool BF16, int NQ, bool TILED>
 hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                      const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
-                     float* Y, int ldy) {
+                     float* Y, int ldy, bool acc_y) {
     const long long slices = (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice;
     const long long blocks = (slices * kWave + kBlock - 1) / kBlock;
     // 16-byte tile loads: every tile pair of a chunk 16-byte aligned (an even
@@ -310,15 +315,18 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
     const unsigned nb = (unsigned)blocks;
     if (long_rows) {
         if (x16)
-            launch_kernel<BF16, NQ, true, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+            launch_kernel<BF16, NQ, true, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
+                                                        acc_y);
         else
-            launch_kernel<BF16, NQ, true, TILED, false>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+            launch_kernel<BF16, NQ, true, TILED, false>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
+                                                        acc_y);
     } else {
         if (x16)
-            launch_kernel<BF16, NQ, false, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+            launch_kernel<BF16, NQ, false, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
+                                                        acc_y);
         else
             launch_kernel<BF16, NQ, false, TILED, false>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y,
-                                                         ldy);
+                                                         ldy, acc_y);
     }
     return hipGetLastError();
 }
@@ -326,7 +334,7 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
 template <bool BF16, bool TILED>
 hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                     const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
-                    float* Y, int ldy) {
+                    float* Y, int ldy, bool acc_y) {
     // Up to 32 right-hand sides per launch (8 quads of accumulators).  With
     // tiled X one launch takes every right-hand side, 16 at a time inside
     // each wave: two register sets of 8 quads would cost the kernel half its
@@ -340,13 +348,13 @@ hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols
         const int nq = (std::min(kc, TILED ? 16 : 32) + 3) / 4;
         hipError_t e;
         if (nq <= 1)
-            e = launch_nq<BF16, 1, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+            e = launch_nq<BF16, 1, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y);
         else if (nq <= 2)
-            e = launch_nq<BF16, 2, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+            e = launch_nq<BF16, 2, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y);
         else if (nq <= 4)
-            e = launch_nq<BF16, 4, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+            e = launch_nq<BF16, 4, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y);
         else if constexpr (!TILED)
-            e = launch_nq<BF16, 8, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy);
+            e = launch_nq<BF16, 8, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y);
         else
             e = hipErrorInvalidValue;  // not reached: tiled chunks hold at most 4 quads
         if (e != hipSuccess) return e;
@@ -374,14 +382,14 @@ namespace launch {
 
 hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows, int cols, int k,
                   const int* rp, const int* bcol, const unsigned short* val,
-                  const unsigned short* X, int ldx, float* Y, int ldy, bool tiled) {
+                  const unsigned short* X, int ldx, float* Y, int ldy, bool tiled, bool accumulate) {
     if (brows <= 0 || k <= 0) return hipSuccess;
     if ((long long)brows * 4 / kBlock + 1 > 0x7FFFFFFFll) return hipErrorInvalidValue;
     if (tiled)
-        return bf16 ? launch_k<true, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy)
-                    : launch_k<false, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
-    return bf16 ? launch_k<true, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy)
-                : launch_k<false, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy);
+        return bf16 ? launch_k<true, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate)
+                    : launch_k<false, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate);
+    return bf16 ? launch_k<true, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate)
+                : launch_k<false, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate);
 }
 
 hipError_t pack_x4(hipStream_t s, int k, int cols, const unsigned short* X, int ldx, unsigned short* X4) {

@@ -159,7 +159,8 @@ constexpr int kSpmmChunk = 16;
 constexpr int kSpmmSlice = 16;  // block rows per slice: one wave, 4 lanes per block row
 hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows, int cols, int k,
                   const int* sptr, const int* scol, const unsigned short* val,
-                  const unsigned short* X, int ldx, float* Y, int ldy, bool tiled = false);
+                  const unsigned short* X, int ldx, float* Y, int ldy, bool tiled = false,
+                  bool accumulate = true);
 // Column-major X (k columns of `cols` values, leading dimension ldx) into
 // the tiled layout above (ceil(cols / 4) * k tiles of 4 values).
 hipError_t pack_x4(hipStream_t s, int k, int cols, const unsigned short* X, int ldx, unsigned short* X4);

@@ -917,6 +917,11 @@ int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, i
 }
 
 int clspmv_spmm_tiled(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X4, float* d_Y, int ldy) {
+    return clspmv_spmm_tiled_ex(ctx, A, k, d_X4, d_Y, ldy, 1);
+}
+
+int clspmv_spmm_tiled_ex(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X4, float* d_Y, int ldy,
+                         int accumulate) {
     if (!ctx || !A || (!d_X4 && k) || (!d_Y && k)) return fail(CLSPMV_ERR_INVALID, "null argument");
     const clspmv_dmat_info& I = A->info;
     if (I.format != CLSPMV_FMT_BSPMM)
@@ -930,7 +935,7 @@ int clspmv_spmm_tiled(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* 
     if (k == 0 || I.rows == 0) return CLSPMV_OK;
     hipError_t e = launch::bspmm4(ctx->stream, A->dtype == CLSPMV_DT_BF16, A->long_brows, A->brows, I.rows, I.cols, k,
                                   A->rp, A->col, A->lpval, static_cast<const unsigned short*>(d_X4), k, d_Y, ldy,
-                                  true);
+                                  true, accumulate != 0);
     if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "spmm launch failed: %s", hipGetErrorString(e));
     return CLSPMV_OK;
 }

@@ -8,8 +8,9 @@
 // takes the matrix's claim (one pass per device matrix across servers),
 // coalesces its waiting requests into one batch, and (3) launches the batch
 // on the matrix's stream: stage each request's x into its columns of the
-// matrix's X, replay the captured (matrix, K) graph (zero y, then one
-// multi-vector pass per 16 columns, one MFMA SpMM, or one SpMV per column),
+// matrix's X, replay the captured (matrix, K) graph (one MFMA SpMM writing
+// y, or zero y, then one multi-vector pass per 16 columns or one SpMV per
+// column),
 // copy each request's y out.  Callers only touch the queue.
 //
 // Bit-for-bit independence from the mix: the kernels of a matrix are fixed
@@ -254,19 +255,20 @@ bool ensure_slot(clspmv_server* S, SMatrix& M) {
     return true;
 }
 
-// Zero y, then the products of K right-hand sides.  Runs on `s` (captured
-// or not).
+// The products of K right-hand sides into y: the SpMM writes Y = A X
+// (the bits a zeroed Y plus A X gives); the other paths add into a zeroed
+// y.  Runs on `s` (captured or not).
 int enqueue_pass(hipStream_t s, const SMatrix& M, int K) {
-    const long long ny = (long long)M.ldy * K;
-    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(ny)), dim3(256), 0, s, ny, M.Y);
-    CLSPMV_HIP(hipGetLastError());
     if (M.path == kSpmm) {
         const clspmv_dmat* A = M.A;
         CLSPMV_HIP(launch::bspmm4(s, A->dtype == CLSPMV_DT_BF16, A->long_brows, A->brows, M.rows, M.cols, K, A->rp,
                                   A->col, A->lpval, static_cast<const unsigned short*>(M.X), K, M.Y, M.ldy,
-                                  /*tiled=*/true));
+                                  /*tiled=*/true, /*accumulate=*/false));
         return CLSPMV_OK;
     }
+    const long long ny = (long long)M.ldy * K;
+    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(ny)), dim3(256), 0, s, ny, M.Y);
+    CLSPMV_HIP(hipGetLastError());
     if (M.path == kMulti) {
         const int kb = M.A->info.format == CLSPMV_FMT_CSR ? kMultiCols : 0;
         for (int c = 0; c * kMultiCols < K; ++c) {

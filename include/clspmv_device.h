@@ -219,6 +219,11 @@ int clspmv_upload_bcsr_lowp(clspmv_ctx* ctx, const clspmv_bcsr_matrix* m, int dt
 int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, int ldx,
                 float* d_Y, int ldy);
 int clspmv_spmm_tiled(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X4, float* d_Y, int ldy);
+/* clspmv_spmm_tiled with accumulate = 1, or, with accumulate = 0, Y = A X:
+ * Y's previous contents are not read (the result equals a zeroed Y plus
+ * A X bit for bit), which saves Y's read traffic. */
+int clspmv_spmm_tiled_ex(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X4, float* d_Y, int ldy,
+                         int accumulate);
 int clspmv_spmm_pack_x(clspmv_ctx* ctx, int k, int cols, const void* d_X, int ldx, void* d_X4);
 int clspmv_convert_from_f32(const float* src, void* dst, size_t n, int dtype);
 int clspmv_convert_to_f32(const void* src, float* dst, size_t n, int dtype);

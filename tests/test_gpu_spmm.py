@@ -141,3 +141,27 @@ def test_spmm_tiled_argument_checks(ctx):
     with pytest.raises(cs.ClspmvError):
         lib_spmm = cs.lib.clspmv_spmm_tiled(ctx.h, dm.h, 4, dX4.p, dY.p, (A.height + 3) // 4 * 4)
         cs.check(lib_spmm, "spmm_tiled")             # not a BSPMM matrix
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("k", [1, 8, 16, 32])
+def test_spmm_tiled_overwrite(ctx, name, k):
+    """clspmv_spmm_tiled_ex with accumulate = 0: Y = A X without reading Y
+    (Y starts as NaN) equals, bit for bit, a zeroed Y plus A X."""
+    A = CASES[name]
+    rng = np.random.default_rng(200 + k)
+    H, W = A.height, A.width
+    ldx, ldy = (W + 3) // 4 * 4, (H + 3) // 4 * 4 + 4
+    X = rng.uniform(-1, 1, size=(k, ldx)).astype(np.float32)
+    Xb = cs.convert_from_f32(X, cs.DT_BF16)
+    M = cs.BlockSpmm(ctx, cs.coo2bcsr(A.copy(), 4, 4), cs.DT_BF16)
+    dX4 = cs.spmm_pack_x(ctx, k, W, cs.DeviceBuffer(ctx, Xb), ldx)
+    nan = np.full((k, ldy), np.nan, np.float32)
+    dY = cs.DeviceBuffer(ctx, nan)
+    dZ = cs.DeviceBuffer(ctx, np.zeros((k, ldy), np.float32))
+    M.spmm_tiled(k, dX4, dY, ldy, accumulate=False)
+    M.spmm_tiled(k, dX4, dZ, ldy)
+    ctx.sync()
+    y, z = dY.download().reshape(k, ldy), dZ.download().reshape(k, ldy)
+    np.testing.assert_array_equal(y[:, :H].view(np.uint32), z[:, :H].view(np.uint32))
+    assert np.all(np.isnan(y[:, H:]))  # past the rows: untouched
