@@ -55,9 +55,9 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
     return v;
 }
 
-// A group of four lanes owns one block row and every right-hand side (in
-// quads of four): each A block is loaded once and multiplied by NQ tiles of
-// X, so A is read once per SpMM whatever k is.  Lane t of a group loads row
+// A group of four lanes owns one block row and NQ quads of four right-hand
+// sides (column-major X: every right-hand side up to 32): each A block it
+// loads is multiplied by NQ tiles of X.  Lane t of a group loads row
 // t of the block and, per quad q, column 4q + t of X.  A wave holds a slice
 // of 16 block rows whose step-j blocks are stored together: its A load of a
 // step is one 512-byte run and its column load one 64-byte run.
@@ -71,10 +71,15 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
 // serves two of them when the stride is even (X16); a right-hand side past
 // k reads some in-bounds tile of the chunk -- its column of the product is
 // never stored, and an MFMA output column depends only on its own B column.
-// NT (TILED, one pass of at most 4 * NQ right-hand sides): A and its block
-// columns are read once, with non-temporal loads, so they do not evict the
-// X tiles that neighbouring slices re-read; with several passes A must stay
-// cached for the later ones and is loaded normally.
+// NT (TILED, at most 4 * NQ right-hand sides): one wave per slice reads A
+// and its block columns once, with non-temporal loads, so they do not evict
+// the X tiles that neighbouring slices re-read.  With more right-hand sides
+// each slice gets one wave per chunk of 4 * NQ of them, the chunks' waves
+// neighbours in one work-group: they run together and read the slice's
+// blocks at about the same time, so A comes from memory about once
+// (1.03 x the algorithmic bytes at k = 32, against 1.41 x for one wave
+// running the chunks one after another, at the same speed); A is then
+// loaded normally so that the neighbours find it in cache.
 template <bool BF16, int NQ, bool LONG, bool TILED, bool X16, bool NT>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int brows, int rows, int cols, int kall, const int* __restrict__ rp,
@@ -85,12 +90,14 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     static_assert(launch::kSpmmSlice * 4 == kWave, "a wave holds one slice, four lanes per block row");
     const int lane = threadIdx.x & (kWave - 1);
     const int t = lane & 3, g = lane >> 2;
-    // One wave per slice, in XCD-aware order: neighbouring block rows, which
-    // share X rows, run on one XCD and share its L2.
-    // (Wave-uniform: in scalar registers, so the loop's exits are scalar
-    // branches.)
-    const int sl = __builtin_amdgcn_readfirstlane(
+    // One wave per slice and chunk of right-hand sides, in XCD-aware order:
+    // neighbouring block rows, which share X rows, run on one XCD and share
+    // its L2.  (Wave-uniform: in scalar registers, so the loop's exits are
+    // scalar branches.)
+    const int nch = (TILED && !NT) ? (kall + 4 * NQ - 1) / (4 * NQ) : 1;
+    const int gw = __builtin_amdgcn_readfirstlane(
         (int)(((long long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) / kWave));
+    const int sl = gw / nch, ch = gw - sl * nch;
     if (sl >= (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice) return;  // the whole wave
     const int br = sl * launch::kSpmmSlice + g;
     const bool active = br < brows;
@@ -243,12 +250,12 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
             if (xc >= k) continue;
             float* yp = Y + (size_t)xc * ldy + r0;
             if (r0 + 3 < rows) {
-                // Several passes over the slice (tiled, more than 4 * NQ
-                // right-hand sides): Y is read and written once per pass,
-                // with non-temporal accesses, so it does not evict the
-                // slice's blocks before the next pass re-reads them.
                 // Y += A X, or Y = A X with Y not read (then 0 + the sum, the
-                // bits a zeroed Y would give).
+                // bits a zeroed Y would give).  With several chunks (tiled,
+                // more than 4 * NQ right-hand sides) Y goes through
+                // non-temporal accesses, which leave the cache to the
+                // slice's blocks that the neighbouring chunks' waves re-read
+                // (3 % faster at k = 32 than plain accesses).
                 f32x4* yq = reinterpret_cast<f32x4*>(yp);
                 f32x4 y = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
                 if (acc_y) {
@@ -266,15 +273,12 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
         }
     };
     if constexpr (TILED) {
-        // 4 * NQ right-hand sides at a time over this slice: the later
-        // passes find the slice's blocks and column indices in cache, so A
-        // is read from memory about once whatever k is.
         if constexpr (NT) {
             run(Xall, Yall, kall);  // kall <= 4 * NQ
             return;
         }
-        for (int c = 0; c < kall; c += 4 * NQ)
-            run(Xall + (size_t)c * 4, Yall + (size_t)c * ldy, min(4 * NQ, kall - c));
+        const int c = ch * 4 * NQ;  // this wave's chunk of right-hand sides
+        run(Xall + (size_t)c * 4, Yall + (size_t)c * ldy, min(4 * NQ, kall - c));
     } else {
         run(Xall, Yall, kall);
     }
@@ -297,19 +301,18 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
                      const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
                      float* Y, int ldy, bool acc_y) {
     const long long slices = (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice;
-    const long long blocks = (slices * kWave + kBlock - 1) / kBlock;
+    const long long chunks = (TILED && k > 4 * NQ) ? (k + 4 * NQ - 1) / (4 * NQ) : 1;  // waves per slice
+    const long long blocks = (slices * chunks * kWave + kBlock - 1) / kBlock;
     // 16-byte tile loads: every tile pair of a chunk 16-byte aligned (an even
     // stride and chunk size -- chunks are 16 right-hand sides but the last,
     // k - 16m, even with k -- and chunks start at multiples of 16) and two
     // of them to load.
     constexpr bool kX16 = TILED && NQ >= 2;
     const bool x16 = kX16 && ldx % 2 == 0 && k % 2 == 0 && ((uintptr_t)X & 15) == 0;
-    // One pass over A (k <= 4 * NQ): non-temporal A loads.  Measured on the
+    // One chunk (k <= 4 * NQ): non-temporal A loads.  Measured on the
     // 524,288 x 12 4x4-block matrix: k = 8 / 16 in 76.6 / 122 us against
-    // 78.5 / 130 us with normal loads; at k = 32 (two passes) normal loads
-    // keep A in cache for the second pass (247 us against 262 us), and
-    // non-temporal Y accesses keep more of it there (236.5 us, 1.30 x the
-    // algorithmic bytes against 1.34 x; the same Y accesses cost 3-5 us at
+    // 78.5 / 130 us with normal loads (Y += A X).  Several chunks: a wave per
+    // chunk, normal A loads, non-temporal Y accesses (which cost 3-5 us at
     // k = 8 / 16).
     const bool nt = k <= 4 * NQ;
     const unsigned nb = (unsigned)blocks;
@@ -336,9 +339,9 @@ hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols
                     const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
                     float* Y, int ldy, bool acc_y) {
     // Up to 32 right-hand sides per launch (8 quads of accumulators).  With
-    // tiled X one launch takes every right-hand side, 16 at a time inside
-    // each wave: two register sets of 8 quads would cost the kernel half its
-    // waves (156 registers against 52 for 4 quads).
+    // tiled X one launch takes every right-hand side, 16 per wave (one pass
+    // of 8 quads, 86 registers, ran at 5 waves per SIMD and 294 against 247
+    // us at k = 32).
     constexpr int kChunk = TILED ? 1 << 30 : 32;
     for (int c = 0; c < k; c += kChunk) {
         const int kc = std::min(kChunk, k - c);
