@@ -155,7 +155,7 @@ class DmatInfo(C.Structure):
                 ("bwidth", C.c_int), ("slice_height", C.c_int), ("panel_width", C.c_int),
                 ("panel_active", C.c_int), ("plain", C.c_int), ("dtype", C.c_int),
                 ("panel_groups", C.c_int), ("panel_subchunks", C.c_int), ("panel_blocks", C.c_int),
-                ("block_slots", C.c_int)]
+                ("block_slots", C.c_int), ("spmm_window", C.c_int)]
 
 
 class DriverReport(C.Structure):

@@ -80,13 +80,31 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
 // (1.03 x the algorithmic bytes at k = 32, against 1.41 x for one wave
 // running the chunks one after another, at the same speed); A is then
 // loaded normally so that the neighbours find it in cache.
-template <bool BF16, int NQ, bool LONG, bool TILED, bool X16, bool NT>
+// WIN (tiled, 16-byte tile loads, no FP64 carries): before its steps a wave
+// copies the tiles of its slice's distinct block columns (the slice's
+// window, win.tile[sl * win.max ..]) for its chunk of right-hand sides into
+// its own win_stride bytes of LDS, kWinTile bytes per tile (then a spare
+// slot that takes the stores of idle lanes and a zero tile for padding
+// slots), and the steps
+// read X from there by each slot's window index (win.idx).  A block column
+// is then fetched once per slice and chunk, with wide contiguous loads,
+// instead of once per step through the texture path, where the tiles of
+// 16 block rows per load cost one cache-line access each and left the
+// steps waiting on L2 (k = 32 ran at 193 us; with every tile load hitting
+// L1 it would run at 126 us, with every load of one line at 84 us).
+template <int NQ>
+constexpr int kWinTile = 8 * 4 * NQ + 16;  // bytes per staged tile: 4 * NQ right-hand sides + 16-byte pad
+
+template <bool BF16, int NQ, bool LONG, bool TILED, bool X16, bool NT, bool WIN>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int brows, int rows, int cols, int kall, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
-    const unsigned short* __restrict__ Xall, int ldx, float* __restrict__ Yall, int ldy, bool acc_y) {
+    const unsigned short* __restrict__ Xall, int ldx, float* __restrict__ Yall, int ldy, bool acc_y,
+    launch::SpmmWindows win, int win_stride) {
     static_assert(!X16 || (TILED && NQ >= 2), "16-byte loads take two tiles of the tiled layout");
     static_assert(!NT || TILED, "non-temporal A loads belong to the tiled kernel");
+    static_assert(!WIN || (X16 && !LONG), "windows are staged with 16-byte loads and carry no FP64 sums");
+    extern __shared__ __align__(16) unsigned char spmm_lds[];
     static_assert(launch::kSpmmSlice * 4 == kWave, "a wave holds one slice, four lanes per block row");
     const int lane = threadIdx.x & (kWave - 1);
     const int t = lane & 3, g = lane >> 2;
@@ -192,27 +210,139 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
                         if (s % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
                     }
                 };
-                // Steps go in pairs with no exit inside a pair (a load used only
-                // past an exit would be moved down to its use): an odd last step
-                // pairs with one past the end, multiplied as zeros (which can
-                // turn a zero sum of -0 into +0; nothing else changes).
-                int cP = ld_col(0), cQ = ld_col(1);
-                s16x4 aP = ld_a(0), aQ = ld_a(1);
-                s16x4 xP[NQ], xQ[NQ];
-                bool uP = cP >= 0, uQ = false;
-                ld_x(cP, xP);
-                cP = ld_col(2);
-                for (int it = 0; it < nmax; it += 2) {
-                    uQ = cQ >= 0 && it + 1 < nmax;
-                    ld_x(cQ, xQ);
-                    cQ = ld_col(it + 3);
-                    mma(it, aP, xP, uP);
-                    aP = ld_a(it + 2);
-                    uP = cP >= 0 && it + 2 < nmax;
+                if constexpr (WIN) {
+                    // Loads from scalar bases with 32-bit lane offsets: the
+                    // steps' address arithmetic stays in scalar registers.
+                    // Two blocks of kD steps' A values and window indices in
+                    // registers, each refilled kD * 2 steps ahead once its
+                    // steps are multiplied; both are in flight while the
+                    // window is staged, so a slice up to 2 kD blocks wide
+                    // waits on memory once.  (At 3 waves per SIMD, which the
+                    // windows' LDS leaves, loads two steps ahead left the
+                    // steps waiting on memory: 111 against 103 us without
+                    // windows at k = 16.)  Padding slots hold zero values and
+                    // the index of a zero tile; steps past the slice's width
+                    // are skipped.
+                    constexpr int kD = 6, PP = 2 * NQ, kR = 3 * NQ;
+                    const char* const a_base = reinterpret_cast<const char*>(val + (size_t)base * 16);
+                    const unsigned char* const i_base = win.idx + base;
+                    const unsigned a_lane = (unsigned)(g * 32 + t * 8);
+                    auto w_step = [&](int s) { return (unsigned)min(s, nmax - 1) * launch::kSpmmSlice; };
+                    auto w_ld_a = [&](int s) {
+                        const s16x4* p = reinterpret_cast<const s16x4*>(a_base + w_step(s) * 32 + a_lane);
+                        if constexpr (NT) return __builtin_nontemporal_load(p);
+                        else return *p;
+                    };
+                    auto w_ld_i = [&](int s) -> int { return i_base[w_step(s) + (unsigned)g]; };
+                    s16x4 aS[2][kD];
+                    int cS[2][kD];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int d = 0; d < kD; ++d) {
+                            cS[h][d] = w_ld_i(h * kD + d);
+                            aS[h][d] = w_ld_a(h * kD + d);
+                        }
+                    // Stage the window: 16-byte piece p of tile i (right-hand
+                    // sides 2p, 2p + 1 of the chunk) to wbuf[i * kWinTile + 16p].
+                    // Every lane loads kR pieces per round, all in flight
+                    // together (one round for up to 96 tiles), and stores
+                    // them without a branch: pieces past the window go to the
+                    // spare tile slot after the largest window, pieces past
+                    // the chunk's k (re-reads of its first piece) stay in
+                    // their tile, where they feed only right-hand sides that
+                    // are never stored.  The slot after the spare is the zero
+                    // tile of padding slots.
+                    unsigned char* const wbuf = spmm_lds + (threadIdx.x / kWave) * win_stride;
+                    const int spare = win.max, zero = win.max + 1;
+                    const int* const w_list = win.tile + (size_t)sl * win.max;
+                    const char* const x_base = reinterpret_cast<const char*>(X);
+                    const unsigned x_tile = (unsigned)ldx * 8;  // bytes per tile
+                    const int npc = win.max * PP;
+                    if (lane < PP)
+                        *reinterpret_cast<s16x8*>(wbuf + zero * kWinTile<NQ> + lane * 16) = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                    for (int i0 = 0; i0 < npc; i0 += kR * kWave) {
+                        int tl[kR];
+#pragma unroll
+                        for (int u = 0; u < kR; ++u)
+                            tl[u] = w_list[(unsigned)min(i0 + u * kWave + lane, npc - 1) / PP];
+                        s16x8 v[kR];
+#pragma unroll
+                        for (int u = 0; u < kR; ++u) {
+                            const unsigned pc = (unsigned)(i0 + u * kWave + lane) % PP;
+                            v[u] = *reinterpret_cast<const s16x8*>(x_base + (unsigned)tl[u] * x_tile +
+                                                                  (2 * pc < (unsigned)k ? pc * 16 : 0u));
+                        }
+#pragma unroll
+                        for (int u = 0; u < kR; ++u) {
+                            const int i = i0 + u * kWave + lane;
+                            const int tile = i < npc ? i / PP : spare;
+                            *reinterpret_cast<s16x8*>(wbuf + tile * kWinTile<NQ> + (i % PP) * 16) = v[u];
+                        }
+                    }
+                    // The wave reads what its own lanes wrote: LDS accesses of
+                    // one wave complete in order; the fence keeps the compiler
+                    // from moving reads above the writes.
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    auto w_ld_x = [&](int c, s16x4* b) {
+                        const unsigned char* wt = wbuf + c * kWinTile<NQ> + t * NQ * 8;
+#pragma unroll
+                        for (int p = 0; p < NQ / 2; ++p) {
+                            const s16x8 v = *reinterpret_cast<const s16x8*>(wt + p * 16);
+                            b[2 * p] = v.lo;
+                            b[2 * p + 1] = v.hi;
+                        }
+                    };
+                    s16x4 xs[2][NQ];
+                    for (int b0 = 0; b0 < nmax; b0 += 2 * kD) {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int sb = b0 + h * kD;
+                            w_ld_x(cS[h][0], xs[0]);
+#pragma unroll
+                            for (int d = 0; d < kD; ++d) {
+                                if (d + 1 < kD) w_ld_x(cS[h][d + 1], xs[(d + 1) & 1]);
+                                if (sb + d < nmax) {  // wave-uniform
+#pragma unroll
+                                    for (int q = 0; q < NQ; ++q)
+                                        acc[q] = mfma4<BF16>(aS[h][d], xs[d & 1][q], acc[q]);
+                                }
+                            }
+                            if (sb + 2 * kD < nmax) {  // wave-uniform
+#pragma unroll
+                                for (int d = 0; d < kD; ++d) {
+                                    cS[h][d] = w_ld_i(sb + 2 * kD + d);
+                                    aS[h][d] = w_ld_a(sb + 2 * kD + d);
+                                }
+                            }
+                        }
+                    }
+                } else {
+                    // Steps go in pairs with no exit inside a pair (a load used
+                    // only past an exit would be moved down to its use): an odd
+                    // last step pairs with one past the end, multiplied as zeros
+                    // (which can turn a zero sum of -0 into +0; nothing else
+                    // changes).
+                    int cP = ld_col(0), cQ = ld_col(1);
+                    s16x4 aP = ld_a(0), aQ = ld_a(1);
+                    s16x4 xP[NQ], xQ[NQ];
+                    bool uP = cP >= 0, uQ = false;
                     ld_x(cP, xP);
-                    cP = ld_col(it + 4);
-                    mma(it + 1, aQ, xQ, uQ);
-                    aQ = ld_a(it + 3);
+                    cP = ld_col(2);
+                    for (int it = 0; it < nmax; it += 2) {
+                        uQ = cQ >= 0 && it + 1 < nmax;
+                        ld_x(cQ, xQ);
+                        cQ = ld_col(it + 3);
+                        mma(it, aP, xP, uP);
+                        aP = ld_a(it + 2);
+                        uP = cP >= 0 && it + 2 < nmax;
+                        ld_x(cP, xP);
+                        cP = ld_col(it + 4);
+                        mma(it + 1, aQ, xQ, uQ);
+                        aQ = ld_a(it + 3);
+                    }
                 }
             }
         } else {
@@ -287,19 +417,33 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
 template <bool BF16, int NQ, bool LONG, bool TILED, bool X16>
 void launch_kernel(hipStream_t s, unsigned blocks, bool nt, int brows, int rows, int cols, int k, const int* rp,
                    const int* bcol, const unsigned short* val, const unsigned short* X, int ldx, float* Y, int ldy,
-                   bool acc_y) {
+                   bool acc_y, const launch::SpmmWindows& win, int win_stride) {
+    if constexpr (TILED && X16 && !LONG) {
+        if (win_stride > 0) {
+            const size_t shm = (size_t)(kBlock / kWave) * win_stride;
+            if (nt)
+                hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, true, true>), dim3(blocks),
+                                   dim3(kBlock), shm, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y,
+                                   win, win_stride);
+            else
+                hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, false, true>), dim3(blocks),
+                                   dim3(kBlock), shm, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y,
+                                   win, win_stride);
+            return;
+        }
+    }
     if (TILED && nt)
-        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, TILED>), dim3(blocks), dim3(kBlock), 0, s,
-                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y);
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, TILED, false>), dim3(blocks), dim3(kBlock), 0,
+                           s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y, win, 0);
     else
-        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, false>), dim3(blocks), dim3(kBlock), 0, s,
-                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y);
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, false, false>), dim3(blocks), dim3(kBlock), 0,
+                           s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y, win, 0);
 }
 
 template <bool BF16, int NQ, bool TILED>
 hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                      const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
-                     float* Y, int ldy, bool acc_y) {
+                     float* Y, int ldy, bool acc_y, const launch::SpmmWindows& win) {
     const long long slices = (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice;
     const long long chunks = (TILED && k > 4 * NQ) ? (k + 4 * NQ - 1) / (4 * NQ) : 1;  // waves per slice
     const long long blocks = (slices * chunks * kWave + kBlock - 1) / kBlock;
@@ -315,21 +459,28 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
     // chunk, normal A loads, non-temporal Y accesses (which cost 3-5 us at
     // k = 8 / 16).
     const bool nt = k <= 4 * NQ;
+    // X windows in LDS when the matrix has them and a work-group's fit.
+    int win_stride = 0;
+    // (The window's X loads take 32-bit byte offsets from X.)
+    if (x16 && !long_rows && win.tile && win.idx && win.max > 0 && win.max <= launch::kSpmmWinMax &&
+        (long long)(kBlock / kWave) * (win.max + 2) * kWinTile<NQ> <= launch::kSpmmWinLds &&
+        (long long)((cols + 3) / 4) * ldx * 8 <= 0xFFFFFFFFll)
+        win_stride = (win.max + 2) * kWinTile<NQ>;
     const unsigned nb = (unsigned)blocks;
     if (long_rows) {
         if (x16)
             launch_kernel<BF16, NQ, true, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
-                                                        acc_y);
+                                                        acc_y, win, 0);
         else
             launch_kernel<BF16, NQ, true, TILED, false>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
-                                                        acc_y);
+                                                        acc_y, win, 0);
     } else {
         if (x16)
             launch_kernel<BF16, NQ, false, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
-                                                        acc_y);
+                                                        acc_y, win, win_stride);
         else
             launch_kernel<BF16, NQ, false, TILED, false>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y,
-                                                         ldy, acc_y);
+                                                         ldy, acc_y, win, 0);
     }
     return hipGetLastError();
 }
@@ -337,7 +488,7 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
 template <bool BF16, bool TILED>
 hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                     const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
-                    float* Y, int ldy, bool acc_y) {
+                    float* Y, int ldy, bool acc_y, const launch::SpmmWindows& win) {
     // Up to 32 right-hand sides per launch (8 quads of accumulators).  With
     // tiled X one launch takes every right-hand side, 16 per wave (one pass
     // of 8 quads, 86 registers, ran at 5 waves per SIMD and 294 against 247
@@ -351,13 +502,13 @@ hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols
         const int nq = (std::min(kc, TILED ? 16 : 32) + 3) / 4;
         hipError_t e;
         if (nq <= 1)
-            e = launch_nq<BF16, 1, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y);
+            e = launch_nq<BF16, 1, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y, win);
         else if (nq <= 2)
-            e = launch_nq<BF16, 2, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y);
+            e = launch_nq<BF16, 2, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y, win);
         else if (nq <= 4)
-            e = launch_nq<BF16, 4, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y);
+            e = launch_nq<BF16, 4, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y, win);
         else if constexpr (!TILED)
-            e = launch_nq<BF16, 8, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y);
+            e = launch_nq<BF16, 8, TILED>(s, long_rows, brows, rows, cols, kc, rp, bcol, val, Xc, ldx, Yc, ldy, acc_y, win);
         else
             e = hipErrorInvalidValue;  // not reached: tiled chunks hold at most 4 quads
         if (e != hipSuccess) return e;
@@ -385,14 +536,20 @@ namespace launch {
 
 hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows, int cols, int k,
                   const int* rp, const int* bcol, const unsigned short* val,
-                  const unsigned short* X, int ldx, float* Y, int ldy, bool tiled, bool accumulate) {
+                  const unsigned short* X, int ldx, float* Y, int ldy, bool tiled, bool accumulate,
+                  SpmmWindows win) {
     if (brows <= 0 || k <= 0) return hipSuccess;
     if ((long long)brows * 4 / kBlock + 1 > 0x7FFFFFFFll) return hipErrorInvalidValue;
+    if (!tiled) win = SpmmWindows{};
     if (tiled)
-        return bf16 ? launch_k<true, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate)
-                    : launch_k<false, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate);
-    return bf16 ? launch_k<true, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate)
-                : launch_k<false, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate);
+        return bf16 ? launch_k<true, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
+                                           accumulate, win)
+                    : launch_k<false, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
+                                            accumulate, win);
+    return bf16 ? launch_k<true, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate,
+                                        win)
+                : launch_k<false, false>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, accumulate,
+                                         win);
 }
 
 hipError_t pack_x4(hipStream_t s, int k, int cols, const unsigned short* X, int ldx, unsigned short* X4) {

@@ -155,12 +155,25 @@ hipError_t csr_multi(hipStream_t s, int nblocks, const int* blocks, const int* r
 // multiples of 4 and X, Y 16-byte aligned.  long_rows: some block row holds
 // more than kLongRowF64 / 4 blocks; the MFMA sums are then carried in FP64
 // every kSpmmChunk blocks.
+// With `win` (tiled X only): each slice's distinct block columns as 4-row
+// tile indices, ascending, at tile[s * max ..] (max per slice, max <=
+// kSpmmWinMax; a slice with fewer repeats its last one), and each slot's
+// index into its slice's list in idx (max + 1 for padding: a zero tile).  A wave then
+// copies its slice's tiles of X into LDS once and reads them from there;
+// used when the windows fit kSpmmWinLds bytes of LDS per work-group.
 constexpr int kSpmmChunk = 16;
 constexpr int kSpmmSlice = 16;  // block rows per slice: one wave, 4 lanes per block row
+constexpr int kSpmmWinMax = 254;
+constexpr int kSpmmWinLds = 64 * 1024;
+struct SpmmWindows {
+    const int* tile = nullptr;
+    const unsigned char* idx = nullptr;
+    int max = 0;
+};
 hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows, int cols, int k,
                   const int* sptr, const int* scol, const unsigned short* val,
                   const unsigned short* X, int ldx, float* Y, int ldy, bool tiled = false,
-                  bool accumulate = true);
+                  bool accumulate = true, SpmmWindows win = {});
 // Column-major X (k columns of `cols` values, leading dimension ldx) into
 // the tiled layout above (ceil(cols / 4) * k tiles of 4 values).
 hipError_t pack_x4(hipStream_t s, int k, int cols, const unsigned short* X, int ldx, unsigned short* X4);

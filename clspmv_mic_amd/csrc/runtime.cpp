@@ -888,6 +888,45 @@ int clspmv_upload_bcsr_lowp(clspmv_ctx* ctx, const clspmv_bcsr_matrix* m, int dt
     UPLOAD_CHECK(upload_array(d, sptr.data(), sptr.size(), &d->rp));
     UPLOAD_CHECK(upload_array(d, scol.data(), scol.size(), &d->col));
     UPLOAD_CHECK(upload_array(d, lp.data(), lp.size(), &d->lpval));
+    // X windows for the tiled kernel: every slice's distinct block columns
+    // (as tile indices) and each slot's index among them, the lists at a
+    // fixed stride (the longest list's length) so that a wave finds its
+    // slice's list without a pointer load; shorter lists repeat their last
+    // tile.  Not built when a slice has more than kSpmmWinMax of them, when
+    // the padding would more than double the tiles staged, nor for
+    // FP64-carried long rows.
+    if (!d->long_brows && nsl > 0) {
+        std::vector<std::vector<int>> lists((size_t)nsl);
+        std::vector<unsigned char> widx(slots, 0xFF);
+        size_t wmax = 0, total = 0;
+        bool ok = true;
+        for (int sl = 0; sl < nsl && ok; ++sl) {
+            std::vector<int>& cur = lists[sl];
+            for (int at = sptr[sl]; at < sptr[sl + 1]; ++at)
+                if (scol[at] >= 0) cur.push_back(scol[at] / 4);
+            std::sort(cur.begin(), cur.end());
+            cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
+            if (cur.size() > (size_t)launch::kSpmmWinMax) ok = false;
+            for (int at = sptr[sl]; at < sptr[sl + 1] && ok; ++at)
+                if (scol[at] >= 0)
+                    widx[at] = (unsigned char)(std::lower_bound(cur.begin(), cur.end(), scol[at] / 4) - cur.begin());
+            wmax = std::max(wmax, cur.size());
+            total += cur.size();
+        }
+        if (ok && wmax > 0 && (size_t)nsl * wmax <= 2 * total + 64) {
+            std::vector<int> wtile((size_t)nsl * wmax);
+            for (int sl = 0; sl < nsl; ++sl) {
+                const std::vector<int>& cur = lists[sl];
+                for (size_t i = 0; i < wmax; ++i)
+                    wtile[(size_t)sl * wmax + i] = cur.empty() ? 0 : cur[std::min(i, cur.size() - 1)];
+            }
+            for (unsigned char& w : widx)
+                if (w == 0xFF) w = (unsigned char)(wmax + 1);  // the zero tile after the spare slot
+            UPLOAD_CHECK(upload_array(d, wtile.data(), wtile.size(), &d->wtile));
+            UPLOAD_CHECK(upload_array(d, widx.data(), widx.size(), &d->widx));
+            d->win_max = (int)wmax;
+        }
+    }
     *out = d;
     return CLSPMV_OK;
 }
@@ -935,7 +974,8 @@ int clspmv_spmm_tiled_ex(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const voi
     if (k == 0 || I.rows == 0) return CLSPMV_OK;
     hipError_t e = launch::bspmm4(ctx->stream, A->dtype == CLSPMV_DT_BF16, A->long_brows, A->brows, I.rows, I.cols, k,
                                   A->rp, A->col, A->lpval, static_cast<const unsigned short*>(d_X4), k, d_Y, ldy,
-                                  true, accumulate != 0);
+                                  true, accumulate != 0,
+                                  launch::SpmmWindows{A->wtile, A->widx, A->win_max});
     if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "spmm launch failed: %s", hipGetErrorString(e));
     return CLSPMV_OK;
 }
@@ -976,6 +1016,7 @@ int clspmv_dmat_get_info(const clspmv_dmat* m, clspmv_dmat_info* out) {
     out->plain = m->plain ? 1 : 0;
     out->dtype = m->dtype;
     out->block_slots = m->bslots;
+    out->spmm_window = m->wtile ? m->win_max : 0;
     return CLSPMV_OK;
 }
 

@@ -47,6 +47,14 @@ struct clspmv_dmat {
     // its (w4, h) float4 at val[((k * bw4 + w4) * bh + h) * hal * 4 + r * 4].
     int bslots = 0;
     bool long_brows = false;  // BSPMM: a block row holds more than kLongRowF64 / 4 blocks
+    // BSPMM X windows (wtile == nullptr: absent): slice s's distinct block
+    // columns as 4-row tile indices, ascending, at wtile[s * win_max ..],
+    // a shorter list repeating its last tile; each slot's index into its
+    // slice's list in widx (win_max + 1 for padding); win_max the longest
+    // list.
+    int* wtile = nullptr;
+    unsigned char* widx = nullptr;
+    int win_max = 0;
     // x-panel encoding of COO / CSR (pw == 0: absent).  plain == false when
     // the (row, col, val) / CSR arrays were not uploaded.
     int pw = 0, pnwg = 0, pnp = 0, pnb = 0, pnsc = 0;

@@ -263,7 +263,8 @@ int enqueue_pass(hipStream_t s, const SMatrix& M, int K) {
         const clspmv_dmat* A = M.A;
         CLSPMV_HIP(launch::bspmm4(s, A->dtype == CLSPMV_DT_BF16, A->long_brows, A->brows, M.rows, M.cols, K, A->rp,
                                   A->col, A->lpval, static_cast<const unsigned short*>(M.X), K, M.Y, M.ldy,
-                                  /*tiled=*/true, /*accumulate=*/false));
+                                  /*tiled=*/true, /*accumulate=*/false,
+                                  launch::SpmmWindows{A->wtile, A->widx, A->win_max}));
         return CLSPMV_OK;
     }
     const long long ny = (long long)M.ldy * K;

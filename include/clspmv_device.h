@@ -93,6 +93,10 @@ typedef struct {
     int    panel_blocks;    /* panel encoding: pass-2 row blocks         */
     int    block_slots;     /* BCSR slot-major encoding: blocks per block
                                row after padding, or 0 (row pointers)     */
+    int    spmm_window;     /* block SpMM: most distinct block columns in
+                               one 16-block-row slice when the X windows
+                               were built (tiled X then staged in LDS),
+                               else 0                                      */
 } clspmv_dmat_info;
 
 /* Upload a host matrix (the reference's cpy_<fmt>_mat,

@@ -103,13 +103,47 @@ def _tiles(Xb: np.ndarray, cols: int) -> np.ndarray:
     return np.ascontiguousarray(full.reshape(k, nt, 4).transpose(1, 0, 2)).reshape(-1)
 
 
+def _window_max(bc) -> int:
+    """Most distinct block columns in one slice of 16 block rows: the X
+    window the tiled kernel stages in LDS.  0 (no windows built) when a slice
+    has more than 254, when padding every slice's list to the longest would
+    more than double the tiles staged, or when a block row is long enough
+    for FP64 carries."""
+    a = bc.arrays()
+    rp, cols = a["rp"], a["col"]
+    nbr = len(rp) - 1
+    if np.any(np.diff(rp) * 4 > 4096):
+        return 0
+    sizes = [len(np.unique(cols[rp[s]:rp[min(nbr, s + 16)]])) for s in range(0, nbr, 16)]
+    if not sizes or max(sizes) > 254 or len(sizes) * max(sizes) > 2 * sum(sizes) + 64:
+        return 0
+    return max(sizes)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_spmm_windows(ctx, name):
+    """The upload builds each slice's X window (its distinct block columns)
+    exactly when every slice has at most 254 and no row needs FP64 carries;
+    the cases hold both kinds."""
+    bc = cs.coo2bcsr(CASES[name].copy(), 4, 4)
+    M = cs.BlockSpmm(ctx, bc, cs.DT_BF16)
+    assert M.info.spmm_window == _window_max(bc)
+    if name in ("block_small", "rand", "rand_dyadic"):
+        assert M.info.spmm_window > 0
+    if name in ("wide", "long_rows"):
+        assert M.info.spmm_window == 0
+
+
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("dtype", [cs.DT_BF16, cs.DT_F16])
-@pytest.mark.parametrize("k", [1, 3, 8, 12, 32, 40])
+@pytest.mark.parametrize("k", [1, 3, 6, 8, 12, 18, 32, 40])
 def test_spmm_tiled_matches_column_major(ctx, name, dtype, k):
     """The tiled-X entry point runs the same products in the same order as
-    clspmv_spmm: identical Y, bit for bit; the GPU packer builds the tiles
-    the host restatement does (odd column counts pad with zeros)."""
+    clspmv_spmm: identical Y, bit for bit, with X read through the LDS
+    windows (even k, every case with windows; chunks narrower than 16 at
+    k = 6, 12, 18, 40) or from the tiles directly (odd k, cases without
+    windows); the GPU packer builds the tiles the host restatement does (odd
+    column counts pad with zeros)."""
     A = CASES[name]
     rng = np.random.default_rng(100 + k)
     H, W = A.height, A.width
