@@ -80,31 +80,13 @@ __device__ __forceinline__ s16x4 ld4(const unsigned short* __restrict__ p, int a
 // (1.03 x the algorithmic bytes at k = 32, against 1.41 x for one wave
 // running the chunks one after another, at the same speed); A is then
 // loaded normally so that the neighbours find it in cache.
-// WIN (tiled, 16-byte tile loads, no FP64 carries): before its steps a wave
-// copies the tiles of its slice's distinct block columns (the slice's
-// window, win.tile[sl * win.max ..]) for its chunk of right-hand sides into
-// its own win_stride bytes of LDS, kWinTile bytes per tile (then a spare
-// slot that takes the stores of idle lanes and a zero tile for padding
-// slots), and the steps
-// read X from there by each slot's window index (win.idx).  A block column
-// is then fetched once per slice and chunk, with wide contiguous loads,
-// instead of once per step through the texture path, where the tiles of
-// 16 block rows per load cost one cache-line access each and left the
-// steps waiting on L2 (k = 32 ran at 193 us; with every tile load hitting
-// L1 it would run at 126 us, with every load of one line at 84 us).
-template <int NQ>
-constexpr int kWinTile = 8 * 4 * NQ + 16;  // bytes per staged tile: 4 * NQ right-hand sides + 16-byte pad
-
-template <bool BF16, int NQ, bool LONG, bool TILED, bool X16, bool NT, bool WIN>
+template <bool BF16, int NQ, bool LONG, bool TILED, bool X16, bool NT>
 __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     int brows, int rows, int cols, int kall, const int* __restrict__ rp,
     const int* __restrict__ bcol, const unsigned short* __restrict__ val,
-    const unsigned short* __restrict__ Xall, int ldx, float* __restrict__ Yall, int ldy, bool acc_y,
-    launch::SpmmWindows win, int win_stride) {
+    const unsigned short* __restrict__ Xall, int ldx, float* __restrict__ Yall, int ldy, bool acc_y) {
     static_assert(!X16 || (TILED && NQ >= 2), "16-byte loads take two tiles of the tiled layout");
     static_assert(!NT || TILED, "non-temporal A loads belong to the tiled kernel");
-    static_assert(!WIN || (X16 && !LONG), "windows are staged with 16-byte loads and carry no FP64 sums");
-    extern __shared__ __align__(16) unsigned char spmm_lds[];
     static_assert(launch::kSpmmSlice * 4 == kWave, "a wave holds one slice, four lanes per block row");
     const int lane = threadIdx.x & (kWave - 1);
     const int t = lane & 3, g = lane >> 2;
@@ -210,139 +192,27 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
                         if (s % launch::kSpmmChunk == launch::kSpmmChunk - 1) carry();  // wave-uniform
                     }
                 };
-                if constexpr (WIN) {
-                    // Loads from scalar bases with 32-bit lane offsets: the
-                    // steps' address arithmetic stays in scalar registers.
-                    // Two blocks of kD steps' A values and window indices in
-                    // registers, each refilled kD * 2 steps ahead once its
-                    // steps are multiplied; both are in flight while the
-                    // window is staged, so a slice up to 2 kD blocks wide
-                    // waits on memory once.  (At 3 waves per SIMD, which the
-                    // windows' LDS leaves, loads two steps ahead left the
-                    // steps waiting on memory: 111 against 103 us without
-                    // windows at k = 16.)  Padding slots hold zero values and
-                    // the index of a zero tile; steps past the slice's width
-                    // are skipped.
-                    constexpr int kD = 6, PP = 2 * NQ, kR = 3 * NQ;
-                    const char* const a_base = reinterpret_cast<const char*>(val + (size_t)base * 16);
-                    const unsigned char* const i_base = win.idx + base;
-                    const unsigned a_lane = (unsigned)(g * 32 + t * 8);
-                    auto w_step = [&](int s) { return (unsigned)min(s, nmax - 1) * launch::kSpmmSlice; };
-                    auto w_ld_a = [&](int s) {
-                        const s16x4* p = reinterpret_cast<const s16x4*>(a_base + w_step(s) * 32 + a_lane);
-                        if constexpr (NT) return __builtin_nontemporal_load(p);
-                        else return *p;
-                    };
-                    auto w_ld_i = [&](int s) -> int { return i_base[w_step(s) + (unsigned)g]; };
-                    s16x4 aS[2][kD];
-                    int cS[2][kD];
-#pragma unroll
-                    for (int h = 0; h < 2; ++h)
-#pragma unroll
-                        for (int d = 0; d < kD; ++d) {
-                            cS[h][d] = w_ld_i(h * kD + d);
-                            aS[h][d] = w_ld_a(h * kD + d);
-                        }
-                    // Stage the window: 16-byte piece p of tile i (right-hand
-                    // sides 2p, 2p + 1 of the chunk) to wbuf[i * kWinTile + 16p].
-                    // Every lane loads kR pieces per round, all in flight
-                    // together (one round for up to 96 tiles), and stores
-                    // them without a branch: pieces past the window go to the
-                    // spare tile slot after the largest window, pieces past
-                    // the chunk's k (re-reads of its first piece) stay in
-                    // their tile, where they feed only right-hand sides that
-                    // are never stored.  The slot after the spare is the zero
-                    // tile of padding slots.
-                    unsigned char* const wbuf = spmm_lds + (threadIdx.x / kWave) * win_stride;
-                    const int spare = win.max, zero = win.max + 1;
-                    const int* const w_list = win.tile + (size_t)sl * win.max;
-                    const char* const x_base = reinterpret_cast<const char*>(X);
-                    const unsigned x_tile = (unsigned)ldx * 8;  // bytes per tile
-                    const int npc = win.max * PP;
-                    if (lane < PP)
-                        *reinterpret_cast<s16x8*>(wbuf + zero * kWinTile<NQ> + lane * 16) = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
-                    for (int i0 = 0; i0 < npc; i0 += kR * kWave) {
-                        int tl[kR];
-#pragma unroll
-                        for (int u = 0; u < kR; ++u)
-                            tl[u] = w_list[(unsigned)min(i0 + u * kWave + lane, npc - 1) / PP];
-                        s16x8 v[kR];
-#pragma unroll
-                        for (int u = 0; u < kR; ++u) {
-                            const unsigned pc = (unsigned)(i0 + u * kWave + lane) % PP;
-                            v[u] = *reinterpret_cast<const s16x8*>(x_base + (unsigned)tl[u] * x_tile +
-                                                                  (2 * pc < (unsigned)k ? pc * 16 : 0u));
-                        }
-#pragma unroll
-                        for (int u = 0; u < kR; ++u) {
-                            const int i = i0 + u * kWave + lane;
-                            const int tile = i < npc ? i / PP : spare;
-                            *reinterpret_cast<s16x8*>(wbuf + tile * kWinTile<NQ> + (i % PP) * 16) = v[u];
-                        }
-                    }
-                    // The wave reads what its own lanes wrote: LDS accesses of
-                    // one wave complete in order; the fence keeps the compiler
-                    // from moving reads above the writes.
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    auto w_ld_x = [&](int c, s16x4* b) {
-                        const unsigned char* wt = wbuf + c * kWinTile<NQ> + t * NQ * 8;
-#pragma unroll
-                        for (int p = 0; p < NQ / 2; ++p) {
-                            const s16x8 v = *reinterpret_cast<const s16x8*>(wt + p * 16);
-                            b[2 * p] = v.lo;
-                            b[2 * p + 1] = v.hi;
-                        }
-                    };
-                    s16x4 xs[2][NQ];
-                    for (int b0 = 0; b0 < nmax; b0 += 2 * kD) {
-#pragma unroll
-                        for (int h = 0; h < 2; ++h) {
-                            const int sb = b0 + h * kD;
-                            w_ld_x(cS[h][0], xs[0]);
-#pragma unroll
-                            for (int d = 0; d < kD; ++d) {
-                                if (d + 1 < kD) w_ld_x(cS[h][d + 1], xs[(d + 1) & 1]);
-                                if (sb + d < nmax) {  // wave-uniform
-#pragma unroll
-                                    for (int q = 0; q < NQ; ++q)
-                                        acc[q] = mfma4<BF16>(aS[h][d], xs[d & 1][q], acc[q]);
-                                }
-                            }
-                            if (sb + 2 * kD < nmax) {  // wave-uniform
-#pragma unroll
-                                for (int d = 0; d < kD; ++d) {
-                                    cS[h][d] = w_ld_i(sb + 2 * kD + d);
-                                    aS[h][d] = w_ld_a(sb + 2 * kD + d);
-                                }
-                            }
-                        }
-                    }
-                } else {
-                    // Steps go in pairs with no exit inside a pair (a load used
-                    // only past an exit would be moved down to its use): an odd
-                    // last step pairs with one past the end, multiplied as zeros
-                    // (which can turn a zero sum of -0 into +0; nothing else
-                    // changes).
-                    int cP = ld_col(0), cQ = ld_col(1);
-                    s16x4 aP = ld_a(0), aQ = ld_a(1);
-                    s16x4 xP[NQ], xQ[NQ];
-                    bool uP = cP >= 0, uQ = false;
+                // Steps go in pairs with no exit inside a pair (a load used only
+                // past an exit would be moved down to its use): an odd last step
+                // pairs with one past the end, multiplied as zeros (which can
+                // turn a zero sum of -0 into +0; nothing else changes).
+                int cP = ld_col(0), cQ = ld_col(1);
+                s16x4 aP = ld_a(0), aQ = ld_a(1);
+                s16x4 xP[NQ], xQ[NQ];
+                bool uP = cP >= 0, uQ = false;
+                ld_x(cP, xP);
+                cP = ld_col(2);
+                for (int it = 0; it < nmax; it += 2) {
+                    uQ = cQ >= 0 && it + 1 < nmax;
+                    ld_x(cQ, xQ);
+                    cQ = ld_col(it + 3);
+                    mma(it, aP, xP, uP);
+                    aP = ld_a(it + 2);
+                    uP = cP >= 0 && it + 2 < nmax;
                     ld_x(cP, xP);
-                    cP = ld_col(2);
-                    for (int it = 0; it < nmax; it += 2) {
-                        uQ = cQ >= 0 && it + 1 < nmax;
-                        ld_x(cQ, xQ);
-                        cQ = ld_col(it + 3);
-                        mma(it, aP, xP, uP);
-                        aP = ld_a(it + 2);
-                        uP = cP >= 0 && it + 2 < nmax;
-                        ld_x(cP, xP);
-                        cP = ld_col(it + 4);
-                        mma(it + 1, aQ, xQ, uQ);
-                        aQ = ld_a(it + 3);
-                    }
+                    cP = ld_col(it + 4);
+                    mma(it + 1, aQ, xQ, uQ);
+                    aQ = ld_a(it + 3);
                 }
             }
         } else {
@@ -414,30 +284,208 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     }
 }
 
+// X windows (tiled X with 16-byte tile loads, no FP64 carries; the upload
+// lists each slice's distinct block columns): one wave per slice, for every
+// chunk of 4 * NQ right-hand sides in turn.  The wave copies the tiles of its
+// slice's window (win.tile[sl * win.max ..]) for the chunk into its own
+// win_stride bytes of LDS, kWinTile bytes per tile -- then a spare slot that
+// takes the stores of idle lanes, a zero tile for padding slots and the
+// window's tile list (4 bytes per tile, for the chunks after the first) -- and the
+// steps read X from there by each slot's window index (win.idx).  A block
+// column is fetched once per slice and chunk, with wide contiguous loads,
+// instead of once per step through the texture path, where the tiles of 16
+// block rows per load cost one cache-line access each and left the steps
+// waiting on L2 (k = 32 ran at 193 us; with every tile load hitting L1 it
+// would run at 126 us, with every load of one line at 84 us).  The slice's A
+// values and window indices are loaded once for all chunks, non-temporally.
+template <int NQ>
+constexpr int kWinTile = 8 * 4 * NQ + 16;  // bytes per staged tile: 4 * NQ right-hand sides + 16-byte pad
+constexpr int kWinBlock = kBlock;  // (one-wave work-groups measured no faster)
+
+// MULTI: more than 4 * NQ right-hand sides (the chunk loop runs more than
+// once).
+template <bool BF16, int NQ, bool MULTI>
+__global__ __launch_bounds__(kWinBlock) __attribute__((amdgpu_waves_per_eu(3))) void bspmm4_win_kernel(
+    int brows, int rows, int kall, const int* __restrict__ rp, const unsigned short* __restrict__ val,
+    launch::SpmmWindows win, int win_stride, const unsigned short* __restrict__ Xall, int ldx,
+    float* __restrict__ Yall, int ldy, bool acc_y) {
+    static_assert(NQ >= 2, "windows are staged in 16-byte pieces of two right-hand sides");
+    // Two blocks of kD steps' A values and window indices in registers, each
+    // refilled kD * 2 steps ahead once its steps are multiplied; both are in
+    // flight while the window is staged, so a slice up to 2 kD blocks wide
+    // waits on memory once.  (At 3 waves per SIMD, which the windows' LDS
+    // leaves, loads two steps ahead left the steps waiting on memory: 111
+    // against 103 us without windows at k = 16.)  The window is staged kR
+    // 16-byte pieces per lane per round, all in flight together (one round
+    // for up to 96 tiles).
+    constexpr int kD = 6, PP = 2 * NQ, kR = 3 * NQ, KC = 4 * NQ;
+    extern __shared__ __align__(16) unsigned char spmm_lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int t = lane & 3, g = lane >> 2;
+    const int sl = __builtin_amdgcn_readfirstlane(
+        (int)(((long long)xcd_tile(blockIdx.x, gridDim.x) * kWinBlock + threadIdx.x) / kWave));
+    if (sl >= (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice) return;  // the whole wave
+    const int br = sl * launch::kSpmmSlice + g;
+    const int base = __builtin_amdgcn_readfirstlane(rp[sl]);
+    const int nmax = __builtin_amdgcn_readfirstlane((rp[sl + 1] - base) / launch::kSpmmSlice);
+    // Loads from scalar bases with 32-bit lane offsets: the steps' address
+    // arithmetic stays in scalar registers.
+    const char* const a_base = reinterpret_cast<const char*>(val + (size_t)base * 16);
+    const unsigned char* const i_base = win.idx + base;
+    const unsigned a_lane = (unsigned)(g * 32 + t * 8);
+    auto step_at = [&](int s) { return (unsigned)min(s, nmax - 1) * launch::kSpmmSlice; };
+    s16x4 aS[2][kD];
+    int cS[2][kD];
+    auto fill = [&](int h, int s0) {
+#pragma unroll
+        for (int d = 0; d < kD; ++d) {
+            cS[h][d] = i_base[step_at(s0 + d) + (unsigned)g];
+            aS[h][d] = __builtin_nontemporal_load(
+                reinterpret_cast<const s16x4*>(a_base + step_at(s0 + d) * 32 + a_lane));
+        }
+    };
+    if (nmax > 0) {
+        fill(0, 0);
+        fill(1, kD);
+    }
+    unsigned char* const wbuf = spmm_lds + (threadIdx.x / kWave) * win_stride;
+    const int spare = win.max, zero = win.max + 1;
+    const int* const w_list = win.tile + (size_t)sl * win.max;
+    const unsigned x_tile = (unsigned)ldx * 8;  // bytes per tile of X
+    const int npc = win.max * PP;
+    int* const wlist_lds = reinterpret_cast<int*>(wbuf + (win.max + 2) * kWinTile<NQ>);
+    if (lane < PP)
+        *reinterpret_cast<s16x8*>(wbuf + zero * kWinTile<NQ> + lane * 16) = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    // Staging of a chunk's window: 16-byte piece p of tile i (right-hand
+    // sides 2p, 2p + 1 of the chunk) to wbuf[i * kWinTile + 16p], kR pieces
+    // per lane per round, stored with no branch: pieces past the window go to
+    // the spare slot, pieces past the chunk's k (re-reads of its first piece)
+    // stay in their tile, where they feed only right-hand sides that are
+    // never stored.  The first chunk reads the window's tile list from memory
+    // and keeps it in LDS (after the zero tile) for the later chunks.
+    auto st_load = [&](int cc, int i0, s16x8* v) {
+        const char* const x_base = reinterpret_cast<const char*>(Xall + (size_t)cc * 4);
+        const int kc = MULTI ? min(KC, kall - cc) : kall;
+        int tl[kR];
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const unsigned j = (unsigned)min(i0 + u * kWave + lane, npc - 1) / PP;
+            tl[u] = (MULTI && cc > 0) ? wlist_lds[j] : w_list[j];
+        }
+        if (MULTI && cc == 0) {
+#pragma unroll
+            for (int u = 0; u < kR; ++u) {
+                const int i = i0 + u * kWave + lane;
+                if (i < npc && i % PP == 0) wlist_lds[i / PP] = tl[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const unsigned pc = (unsigned)(i0 + u * kWave + lane) % PP;
+            v[u] = *reinterpret_cast<const s16x8*>(x_base + (unsigned)tl[u] * x_tile +
+                                                  (2 * pc < (unsigned)kc ? pc * 16 : 0u));
+        }
+    };
+    auto st_store = [&](int i0, const s16x8* v) {
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const int i = i0 + u * kWave + lane;
+            const int tile = i < npc ? i / PP : spare;
+            *reinterpret_cast<s16x8*>(wbuf + tile * kWinTile<NQ> + (i % PP) * 16) = v[u];
+        }
+    };
+    // The wave reads only what its own lanes wrote and its LDS accesses
+    // complete in order; the fences keep the compiler from moving a chunk's
+    // reads across the writes of the window around them.
+    auto lds_fence = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    for (int c = 0; c < (MULTI ? kall : 1); c += KC) {
+        const int k = MULTI ? min(KC, kall - c) : kall;
+        f32x4 acc[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        if (nmax > 0) {
+            lds_fence();
+            for (int i0 = 0; i0 < npc; i0 += kR * kWave) {
+                s16x8 v[kR];
+                st_load(c, i0, v);
+                st_store(i0, v);
+            }
+            lds_fence();
+            // Lane t of a group takes right-hand sides t * NQ + q.
+            auto ld_x = [&](int w, s16x4* b) {
+                const unsigned char* wt = wbuf + w * kWinTile<NQ> + t * NQ * 8;
+#pragma unroll
+                for (int p = 0; p < NQ / 2; ++p) {
+                    const s16x8 v = *reinterpret_cast<const s16x8*>(wt + p * 16);
+                    b[2 * p] = v.lo;
+                    b[2 * p + 1] = v.hi;
+                }
+            };
+            // Padding slots hold zero values and the zero tile's index; steps
+            // past the slice's width are skipped, so the products and their
+            // order are those of the column-major kernel.
+            s16x4 xs[2][NQ];
+            for (int b0 = 0; b0 < nmax; b0 += 2 * kD) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int sb = b0 + h * kD;
+                    ld_x(cS[h][0], xs[0]);
+#pragma unroll
+                    for (int d = 0; d < kD; ++d) {
+                        if (d + 1 < kD) ld_x(cS[h][d + 1], xs[(d + 1) & 1]);
+                        if (sb + d < nmax) {  // wave-uniform
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) acc[q] = mfma4<BF16>(aS[h][d], xs[d & 1][q], acc[q]);
+                        }
+                    }
+                    // Refill kD * 2 steps ahead; in a slice wider than 2 kD
+                    // steps, the last refills load the first blocks again for
+                    // the next chunk.  (Wave-uniform.)
+                    if (sb + 2 * kD < nmax || (MULTI && nmax > 2 * kD && c + KC < kall))
+                        fill(h, sb + 2 * kD < nmax ? sb + 2 * kD : h * kD);
+                }
+            }
+        }
+        // Y += A X, or Y = A X with Y not read (then 0 + the sum, the bits a
+        // zeroed Y would give).
+        if (br < brows) {
+            const int r0 = br * 4;
+            float* const Y = Yall + (size_t)c * ldy;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int xc = t * NQ + q;
+                if (xc >= k) continue;
+                float* yp = Y + (size_t)xc * ldy + r0;
+                if (r0 + 3 < rows) {
+                    f32x4* yq = reinterpret_cast<f32x4*>(yp);
+                    f32x4 y = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                    if (acc_y) y = *yq;
+                    y += acc[q];
+                    *yq = y;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (r0 + i < rows) yp[i] = (acc_y ? yp[i] : 0.0f) + acc[q][i];
+                }
+            }
+        }
+    }
+}
+
 template <bool BF16, int NQ, bool LONG, bool TILED, bool X16>
 void launch_kernel(hipStream_t s, unsigned blocks, bool nt, int brows, int rows, int cols, int k, const int* rp,
                    const int* bcol, const unsigned short* val, const unsigned short* X, int ldx, float* Y, int ldy,
-                   bool acc_y, const launch::SpmmWindows& win, int win_stride) {
-    if constexpr (TILED && X16 && !LONG) {
-        if (win_stride > 0) {
-            const size_t shm = (size_t)(kBlock / kWave) * win_stride;
-            if (nt)
-                hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, true, true>), dim3(blocks),
-                                   dim3(kBlock), shm, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y,
-                                   win, win_stride);
-            else
-                hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, false, true>), dim3(blocks),
-                                   dim3(kBlock), shm, s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y,
-                                   win, win_stride);
-            return;
-        }
-    }
+                   bool acc_y) {
     if (TILED && nt)
-        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, TILED, false>), dim3(blocks), dim3(kBlock), 0,
-                           s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y, win, 0);
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, TILED>), dim3(blocks), dim3(kBlock), 0, s,
+                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y);
     else
-        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, false, false>), dim3(blocks), dim3(kBlock), 0,
-                           s, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y, win, 0);
+        hipLaunchKernelGGL((bspmm4_kernel<BF16, NQ, LONG, TILED, X16, false>), dim3(blocks), dim3(kBlock), 0, s,
+                           brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y);
 }
 
 template <bool BF16, int NQ, bool TILED>
@@ -459,28 +507,40 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
     // chunk, normal A loads, non-temporal Y accesses (which cost 3-5 us at
     // k = 8 / 16).
     const bool nt = k <= 4 * NQ;
-    // X windows in LDS when the matrix has them and a work-group's fit.
-    int win_stride = 0;
-    // (The window's X loads take 32-bit byte offsets from X.)
+    // X windows in LDS when the matrix has them and a work-group's fit (the
+    // window's X loads take 32-bit byte offsets from X): one wave per slice
+    // for all chunks.
     if (x16 && !long_rows && win.tile && win.idx && win.max > 0 && win.max <= launch::kSpmmWinMax &&
-        (long long)(kBlock / kWave) * (win.max + 2) * kWinTile<NQ> <= launch::kSpmmWinLds &&
-        (long long)((cols + 3) / 4) * ldx * 8 <= 0xFFFFFFFFll)
-        win_stride = (win.max + 2) * kWinTile<NQ>;
+        (long long)(kWinBlock / kWave) * ((win.max + 2) * kWinTile<NQ> + win.max * 4 + 15) <= launch::kSpmmWinLds &&
+        (long long)((cols + 3) / 4) * ldx * 8 <= 0xFFFFFFFFll) {
+        if constexpr (kX16) {
+            const int win_stride = (win.max + 2) * kWinTile<NQ> + (win.max * 4 + 15) / 16 * 16;
+            const unsigned wb = (unsigned)((slices * kWave + kWinBlock - 1) / kWinBlock);
+            const size_t shm = (size_t)(kWinBlock / kWave) * win_stride;
+            if (k > 4 * NQ)
+                hipLaunchKernelGGL((bspmm4_win_kernel<BF16, NQ, true>), dim3(wb), dim3(kWinBlock), shm, s, brows, rows, k,
+                                   rp, val, win, win_stride, X, ldx, Y, ldy, acc_y);
+            else
+                hipLaunchKernelGGL((bspmm4_win_kernel<BF16, NQ, false>), dim3(wb), dim3(kWinBlock), shm, s, brows, rows,
+                                   k, rp, val, win, win_stride, X, ldx, Y, ldy, acc_y);
+            return hipGetLastError();
+        }
+    }
     const unsigned nb = (unsigned)blocks;
     if (long_rows) {
         if (x16)
             launch_kernel<BF16, NQ, true, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
-                                                        acc_y, win, 0);
+                                                        acc_y);
         else
             launch_kernel<BF16, NQ, true, TILED, false>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
-                                                        acc_y, win, 0);
+                                                        acc_y);
     } else {
         if (x16)
             launch_kernel<BF16, NQ, false, TILED, kX16>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
-                                                        acc_y, win, win_stride);
+                                                        acc_y);
         else
             launch_kernel<BF16, NQ, false, TILED, false>(s, nb, nt, brows, rows, cols, k, rp, bcol, val, X, ldx, Y,
-                                                         ldy, acc_y, win, 0);
+                                                         ldy, acc_y);
     }
     return hipGetLastError();
 }
