@@ -353,13 +353,13 @@ def run_spmm(args):
                          "unit": "GB/s", "frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
                          # counters of the BF16 tiled kernel at this k, where measured
                          "traffic": measured_traffic(f"spmm-set-k{k}" if overwrite else f"spmm-k{k}",
-                                                     "bspmm4_kernel")
+                                                     "bspmm4")
                          if tiled and args.dtype == "bf16" else None,
                          "algorithmic_bytes": nbytes},
             "y_accumulate": None if acc_ms is None else {
                 "ms": round(acc_ms, 6), "algorithmic_bytes": acc_bytes,
                 "frac": round(acc_bytes / (acc_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4),
-                "traffic": measured_traffic(f"spmm-k{k}", "bspmm4_kernel") if args.dtype == "bf16" else None},
+                "traffic": measured_traffic(f"spmm-k{k}", "bspmm4") if args.dtype == "bf16" else None},
             "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
             "column_major_x_ms": round(other_ms, 6) if other_ms is not None else None,
             "error_vs_fp64_stored_values": [float(f"{e:.3e}") for e in errs],
