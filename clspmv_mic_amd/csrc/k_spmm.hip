@@ -21,6 +21,7 @@
 // (blocks in storage order, four products per instruction) is fixed.
 
 #include <algorithm>
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -284,8 +285,8 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
     }
 }
 
-// X windows (tiled X with 16-byte tile loads, no FP64 carries; the upload
-// lists each slice's distinct block columns): one wave per slice, for every
+// X windows (no FP64 carries; the upload lists each slice's distinct block
+// columns): one wave per slice, for every
 // chunk of 4 * NQ right-hand sides in turn.  The wave copies the tiles of its
 // slice's window (win.tile[sl * win.max ..]) for the chunk into its own
 // win_stride bytes of LDS, kWinTile bytes per tile -- then a spare slot that
@@ -298,26 +299,29 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
 // waiting on L2 (k = 32 ran at 193 us; with every tile load hitting L1 it
 // would run at 126 us, with every load of one line at 84 us).  The slice's A
 // values and window indices are loaded once for all chunks, non-temporally.
+// X comes in 4-row tiles (16-byte pieces: two right-hand sides of a tile)
+// or, with COLX, column-major (8-byte pieces: the tile's four rows of one
+// right-hand side, read from its column; rows past `cols` staged as zeros).
 template <int NQ>
 constexpr int kWinTile = 8 * 4 * NQ + 16;  // bytes per staged tile: 4 * NQ right-hand sides + 16-byte pad
 constexpr int kWinBlock = kBlock;  // (one-wave work-groups measured no faster)
 
 // MULTI: more than 4 * NQ right-hand sides (the chunk loop runs more than
 // once).
-template <bool BF16, int NQ, bool MULTI>
+template <bool BF16, int NQ, bool MULTI, bool COLX>
 __global__ __launch_bounds__(kWinBlock) __attribute__((amdgpu_waves_per_eu(3))) void bspmm4_win_kernel(
-    int brows, int rows, int kall, const int* __restrict__ rp, const unsigned short* __restrict__ val,
+    int brows, int rows, int cols, int kall, const int* __restrict__ rp, const unsigned short* __restrict__ val,
     launch::SpmmWindows win, int win_stride, const unsigned short* __restrict__ Xall, int ldx,
     float* __restrict__ Yall, int ldy, bool acc_y) {
-    static_assert(NQ >= 2, "windows are staged in 16-byte pieces of two right-hand sides");
+    static_assert(NQ >= 2, "a lane reads the window in 16-byte pieces of two right-hand sides");
     // Two blocks of kD steps' A values and window indices in registers, each
     // refilled kD * 2 steps ahead once its steps are multiplied; both are in
     // flight while the window is staged, so a slice up to 2 kD blocks wide
     // waits on memory once.  (At 3 waves per SIMD, which the windows' LDS
     // leaves, loads two steps ahead left the steps waiting on memory: 111
     // against 103 us without windows at k = 16.)  The window is staged kR
-    // 16-byte pieces per lane per round, all in flight together (one round
-    // for up to 96 tiles).
+    // pieces per lane per round, all in flight together (one round for up to
+    // 96 tiles).
     constexpr int kD = 6, PP = 2 * NQ, kR = 3 * NQ, KC = 4 * NQ;
     extern __shared__ __align__(16) unsigned char spmm_lds[];
     const int lane = threadIdx.x & (kWave - 1);
@@ -356,13 +360,14 @@ __global__ __launch_bounds__(kWinBlock) __attribute__((amdgpu_waves_per_eu(3))) 
     int* const wlist_lds = reinterpret_cast<int*>(wbuf + (win.max + 2) * kWinTile<NQ>);
     if (lane < PP)
         *reinterpret_cast<s16x8*>(wbuf + zero * kWinTile<NQ> + lane * 16) = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
-    // Staging of a chunk's window: 16-byte piece p of tile i (right-hand
-    // sides 2p, 2p + 1 of the chunk) to wbuf[i * kWinTile + 16p], kR pieces
-    // per lane per round, stored with no branch: pieces past the window go to
-    // the spare slot, pieces past the chunk's k (re-reads of its first piece)
-    // stay in their tile, where they feed only right-hand sides that are
-    // never stored.  The first chunk reads the window's tile list from memory
-    // and keeps it in LDS (after the zero tile) for the later chunks.
+    // Staging of a chunk's window from tiled X: 16-byte piece p of tile i
+    // (right-hand sides 2p, 2p + 1 of the chunk) to wbuf[i * kWinTile + 16p],
+    // kR pieces per lane per round, stored with no branch: pieces past the
+    // window go to the spare slot, pieces past the chunk's k (re-reads of its
+    // first piece) stay in their tile, where they feed only right-hand sides
+    // that are never stored.  The first chunk reads the window's tile list
+    // from memory and keeps it in LDS (after the zero tile) for the later
+    // chunks.
     auto st_load = [&](int cc, int i0, s16x8* v) {
         const char* const x_base = reinterpret_cast<const char*>(Xall + (size_t)cc * 4);
         const int kc = MULTI ? min(KC, kall - cc) : kall;
@@ -394,6 +399,37 @@ __global__ __launch_bounds__(kWinBlock) __attribute__((amdgpu_waves_per_eu(3))) 
             *reinterpret_cast<s16x8*>(wbuf + tile * kWinTile<NQ> + (i % PP) * 16) = v[u];
         }
     };
+    // Staging from column-major X: lane L of a round takes window tile j0 + L
+    // and loads its four rows from the column of each of the chunk's
+    // right-hand sides (in bounds: ldx is a multiple of 4 at least cols), so
+    // one load reads one column at the window's tiles -- runs of consecutive
+    // tiles, a few cache lines -- and stores them at wbuf[j * kWinTile + 8 r]
+    // (the spare slot past the window); rows past `cols` are staged as zeros.
+    auto st_colx = [&](int cc) {
+        const char* const x_base = reinterpret_cast<const char*>(Xall + (size_t)cc * ldx);
+        const int kc = MULTI ? min(KC, kall - cc) : kall;
+        for (int j0 = 0; j0 < win.max; j0 += kWave) {
+            const int j = j0 + lane, jc = min(j, win.max - 1);
+            const int tl = (MULTI && cc > 0) ? wlist_lds[jc] : w_list[jc];
+            if (MULTI && cc == 0 && j < win.max) wlist_lds[j] = tl;
+            s16x4 v[KC];
+#pragma unroll
+            for (int r = 0; r < KC; ++r)
+                v[r] = *reinterpret_cast<const s16x4*>(x_base +
+                                                      ((r < kc ? (unsigned)r : 0u) * (unsigned)ldx + (unsigned)tl * 4) * 2);
+            if (cols & 3) {  // wave-uniform: zeros past the last column
+                const int past = tl * 4 - cols;  // rows 4 tl + e with e >= -past are past
+#pragma unroll
+                for (int r = 0; r < KC; ++r)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (e + past >= 0) v[r][e] = 0;
+            }
+            unsigned char* const dst = wbuf + (j < win.max ? j : spare) * kWinTile<NQ>;
+#pragma unroll
+            for (int r = 0; r < KC; ++r) *reinterpret_cast<s16x4*>(dst + r * 8) = v[r];
+        }
+    };
     // The wave reads only what its own lanes wrote and its LDS accesses
     // complete in order; the fences keep the compiler from moving a chunk's
     // reads across the writes of the window around them.
@@ -409,10 +445,14 @@ __global__ __launch_bounds__(kWinBlock) __attribute__((amdgpu_waves_per_eu(3))) 
         for (int q = 0; q < NQ; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         if (nmax > 0) {
             lds_fence();
-            for (int i0 = 0; i0 < npc; i0 += kR * kWave) {
-                s16x8 v[kR];
-                st_load(c, i0, v);
-                st_store(i0, v);
+            if constexpr (COLX) {
+                st_colx(c);
+            } else {
+                for (int i0 = 0; i0 < npc; i0 += kR * kWave) {
+                    s16x8 v[kR];
+                    st_load(c, i0, v);
+                    st_store(i0, v);
+                }
             }
             lds_fence();
             // Lane t of a group takes right-hand sides t * NQ + q.
@@ -488,6 +528,32 @@ void launch_kernel(hipStream_t s, unsigned blocks, bool nt, int brows, int rows,
                            brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy, acc_y);
 }
 
+// The window kernel, when the matrix has X windows, its rows need no FP64
+// carries, a work-group's windows fit kSpmmWinLds and X's byte offsets fit
+// 32 bits (the kernel's staging loads take 32-bit offsets from X); sets
+// `done` when it launched.  One wave per slice for all k right-hand sides.
+template <bool BF16, int NQ, bool COLX>
+hipError_t launch_win(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
+                      const unsigned short* val, const unsigned short* X, int ldx, float* Y, int ldy, bool acc_y,
+                      const launch::SpmmWindows& win, bool& done) {
+    done = false;
+    if (long_rows || !win.tile || !win.idx || win.max <= 0 || win.max > launch::kSpmmWinMax) return hipSuccess;
+    const long long x_bytes = COLX ? (long long)k * ldx * 2 : (long long)((cols + 3) / 4) * ldx * 8;
+    const int win_stride = (win.max + 2) * kWinTile<NQ> + (win.max * 4 + 15) / 16 * 16;
+    if (x_bytes > 0xFFFFFFFFll || (long long)(kWinBlock / kWave) * win_stride > launch::kSpmmWinLds) return hipSuccess;
+    const long long slices = (brows + launch::kSpmmSlice - 1) / launch::kSpmmSlice;
+    const unsigned wb = (unsigned)((slices * kWave + kWinBlock - 1) / kWinBlock);
+    const size_t shm = (size_t)(kWinBlock / kWave) * win_stride;
+    if (k > 4 * NQ)
+        hipLaunchKernelGGL((bspmm4_win_kernel<BF16, NQ, true, COLX>), dim3(wb), dim3(kWinBlock), shm, s, brows, rows,
+                           cols, k, rp, val, win, win_stride, X, ldx, Y, ldy, acc_y);
+    else
+        hipLaunchKernelGGL((bspmm4_win_kernel<BF16, NQ, false, COLX>), dim3(wb), dim3(kWinBlock), shm, s, brows,
+                           rows, cols, k, rp, val, win, win_stride, X, ldx, Y, ldy, acc_y);
+    done = true;
+    return hipGetLastError();
+}
+
 template <bool BF16, int NQ, bool TILED>
 hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int cols, int k, const int* rp,
                      const int* bcol, const unsigned short* val, const unsigned short* X, int ldx,
@@ -507,23 +573,13 @@ hipError_t launch_nq(hipStream_t s, bool long_rows, int brows, int rows, int col
     // chunk, normal A loads, non-temporal Y accesses (which cost 3-5 us at
     // k = 8 / 16).
     const bool nt = k <= 4 * NQ;
-    // X windows in LDS when the matrix has them and a work-group's fit (the
-    // window's X loads take 32-bit byte offsets from X): one wave per slice
-    // for all chunks.
-    if (x16 && !long_rows && win.tile && win.idx && win.max > 0 && win.max <= launch::kSpmmWinMax &&
-        (long long)(kWinBlock / kWave) * ((win.max + 2) * kWinTile<NQ> + win.max * 4 + 15) <= launch::kSpmmWinLds &&
-        (long long)((cols + 3) / 4) * ldx * 8 <= 0xFFFFFFFFll) {
-        if constexpr (kX16) {
-            const int win_stride = (win.max + 2) * kWinTile<NQ> + (win.max * 4 + 15) / 16 * 16;
-            const unsigned wb = (unsigned)((slices * kWave + kWinBlock - 1) / kWinBlock);
-            const size_t shm = (size_t)(kWinBlock / kWave) * win_stride;
-            if (k > 4 * NQ)
-                hipLaunchKernelGGL((bspmm4_win_kernel<BF16, NQ, true>), dim3(wb), dim3(kWinBlock), shm, s, brows, rows, k,
-                                   rp, val, win, win_stride, X, ldx, Y, ldy, acc_y);
-            else
-                hipLaunchKernelGGL((bspmm4_win_kernel<BF16, NQ, false>), dim3(wb), dim3(kWinBlock), shm, s, brows, rows,
-                                   k, rp, val, win, win_stride, X, ldx, Y, ldy, acc_y);
-            return hipGetLastError();
+    // X windows in LDS when the matrix has them (16-byte tile pieces).
+    if constexpr (kX16) {
+        if (x16) {
+            bool done = false;
+            const hipError_t e = launch_win<BF16, NQ, false>(s, long_rows, brows, rows, cols, k, rp, val, X, ldx, Y,
+                                                             ldy, acc_y, win, done);
+            if (done || e != hipSuccess) return e;
         }
     }
     const unsigned nb = (unsigned)blocks;
@@ -554,6 +610,17 @@ hipError_t launch_k(hipStream_t s, bool long_rows, int brows, int rows, int cols
     // of 8 quads, 86 registers, ran at 5 waves per SIMD and 294 against 247
     // us at k = 32).
     constexpr int kChunk = TILED ? 1 << 30 : 32;
+    if constexpr (!TILED) {
+        // X windows staged from the columns of X: one launch for every k,
+        // 8 or 16 right-hand sides per chunk.
+        bool done = false;
+        const hipError_t e =
+            k <= 8 ? launch_win<BF16, 2, true>(s, long_rows, brows, rows, cols, k, rp, val, X, ldx, Y, ldy, acc_y, win,
+                                               done)
+                   : launch_win<BF16, 4, true>(s, long_rows, brows, rows, cols, k, rp, val, X, ldx, Y, ldy, acc_y, win,
+                                               done);
+        if (done || e != hipSuccess) return e;
+    }
     for (int c = 0; c < k; c += kChunk) {
         const int kc = std::min(kChunk, k - c);
         // Right-hand side c: column c, or offset c inside every tile.
@@ -600,7 +667,6 @@ hipError_t bspmm4(hipStream_t s, bool bf16, bool long_rows, int brows, int rows,
                   SpmmWindows win) {
     if (brows <= 0 || k <= 0) return hipSuccess;
     if ((long long)brows * 4 / kBlock + 1 > 0x7FFFFFFFll) return hipErrorInvalidValue;
-    if (!tiled) win = SpmmWindows{};
     if (tiled)
         return bf16 ? launch_k<true, true>(s, long_rows, brows, rows, cols, k, rp, bcol, val, X, ldx, Y, ldy,
                                            accumulate, win)

@@ -950,7 +950,8 @@ int clspmv_spmm(clspmv_ctx* ctx, const clspmv_dmat* A, int k, const void* d_X, i
     if (k == 0 || I.rows == 0) return CLSPMV_OK;
     hipError_t e = launch::bspmm4(ctx->stream, A->dtype == CLSPMV_DT_BF16, A->long_brows, A->brows, I.rows, I.cols, k,
                                   A->rp, A->col, A->lpval, static_cast<const unsigned short*>(d_X), ldx,
-                                  d_Y, ldy);
+                                  d_Y, ldy, /*tiled=*/false, /*accumulate=*/true,
+                                  launch::SpmmWindows{A->wtile, A->widx, A->win_max});
     if (e != hipSuccess) return fail(CLSPMV_ERR_HIP, "spmm launch failed: %s", hipGetErrorString(e));
     return CLSPMV_OK;
 }

@@ -592,7 +592,8 @@ int run_plain(clspmv_server* S, SMatrix& M, const clspmv_request& r, float* pinn
         const clspmv_dmat* A = P.A;
         hipError_t e = launch::bspmm4(s, A->dtype == CLSPMV_DT_BF16, A->long_brows, A->brows, P.rows, P.cols, r.k,
                                       A->rp, A->col, A->lpval, static_cast<const unsigned short*>(P.X), P.ldx, P.Y,
-                                      P.ldy);
+                                      P.ldy, /*tiled=*/false, /*accumulate=*/true,
+                                      launch::SpmmWindows{A->wtile, A->widx, A->win_max});
         if (e != hipSuccess) st = fail(CLSPMV_ERR_HIP, "spmm launch failed: %s", hipGetErrorString(e));
     } else {
         for (int j = 0; j < r.k && !st; ++j)

@@ -75,6 +75,33 @@ def test_spmm_leading_dimensions(ctx, oracle):
     _run(ctx, oracle, CASES["rand"], cs.DT_BF16, 8, pad=12)
 
 
+@pytest.mark.parametrize("k", [3, 8, 20])
+def test_spmm_column_major_padding_unused(ctx, k):
+    """Column-major X: the rows past the matrix's width (333 columns, so the
+    last 4-row tile is partial) are never used, even when they hold NaN --
+    the X windows stage them as zeros; Y is bit for bit the one of X with
+    zeros there."""
+    A = CASES["rand"]
+    rng = np.random.default_rng(300 + k)
+    H, W = A.height, A.width
+    ldx, ldy = (W + 3) // 4 * 4 + 4, (H + 3) // 4 * 4
+    X = rng.uniform(-1, 1, size=(k, ldx)).astype(np.float32)
+    X[:, W:] = 0.0
+    Xn = X.copy()
+    Xn[:, W:] = np.nan
+    M = cs.BlockSpmm(ctx, cs.coo2bcsr(A.copy(), 4, 4), cs.DT_BF16)
+    assert M.info.spmm_window > 0
+    ys = []
+    for Xi in (X, Xn):
+        dX = cs.DeviceBuffer(ctx, cs.convert_from_f32(Xi, cs.DT_BF16))
+        dY = cs.DeviceBuffer(ctx, np.zeros((k, ldy), np.float32))
+        M.spmm(k, dX, ldx, dY, ldy)
+        ctx.sync()
+        ys.append(dY.download())
+    assert not np.any(np.isnan(ys[1]))
+    np.testing.assert_array_equal(ys[1].view(np.uint32), ys[0].view(np.uint32))
+
+
 def test_spmm_argument_checks(ctx):
     A = CASES["rand"]
     M = cs.BlockSpmm(ctx, cs.coo2bcsr(A.copy(), 4, 4), cs.DT_BF16)
