@@ -25,6 +25,11 @@ def _cases():
         "rand_dyadic": matrices.random_sparse(300, 260, 0.05, seed=42, dyadic=True),
         "wide": matrices.random_sparse(37, 1201, 0.03, seed=43),
         "long_rows": matrices.random_sparse(64, 3000, 0.002, seed=44, long_rows=3, long_len=2500),
+        # X windows of 101 tiles (two staging rounds at 16 right-hand sides)
+        # and of 168 (windows at k <= 8 only, two or three rounds; the LDS
+        # of 16 right-hand sides holds at most 108 tiles)
+        "win_rounds": matrices.random_sparse(256, 4000, 0.0004, seed=50),
+        "win_wide": matrices.random_sparse(256, 4000, 0.0007, seed=45),
     }
 
 
@@ -157,6 +162,10 @@ def test_spmm_windows(ctx, name):
     assert M.info.spmm_window == _window_max(bc)
     if name in ("block_small", "rand", "rand_dyadic"):
         assert M.info.spmm_window > 0
+    if name == "win_rounds":
+        assert 96 < M.info.spmm_window <= 108
+    if name == "win_wide":
+        assert M.info.spmm_window > 108
     if name in ("wide", "long_rows"):
         assert M.info.spmm_window == 0
 
