@@ -286,13 +286,13 @@ __global__ __launch_bounds__(kBlock) void bspmm4_kernel(
 }
 
 // X windows (no FP64 carries; the upload lists each slice's distinct block
-// columns): one wave per slice, for every
-// chunk of 4 * NQ right-hand sides in turn.  The wave copies the tiles of its
-// slice's window (win.tile[sl * win.max ..]) for the chunk into its own
-// win_stride bytes of LDS, kWinTile bytes per tile -- then a spare slot that
-// takes the stores of idle lanes, a zero tile for padding slots and the
-// window's tile list (4 bytes per tile, for the chunks after the first) -- and the
-// steps read X from there by each slot's window index (win.idx).  A block
+// columns): one wave per slice, for every chunk of 4 * NQ right-hand sides
+// in turn.  The wave copies the tiles of its slice's window
+// (win.tile[sl * win.max ..]) for the chunk into its own win_stride bytes of
+// LDS, kWinTile bytes per tile -- then a spare slot that takes the stores of
+// idle lanes, a zero tile for padding slots and the window's tile list (4
+// bytes per tile, for the chunks after the first) -- and the steps read X
+// from there by each slot's window index (win.idx).  A block
 // column is fetched once per slice and chunk, with wide contiguous loads,
 // instead of once per step through the texture path, where the tiles of 16
 // block rows per load cost one cache-line access each and left the steps
@@ -319,9 +319,10 @@ __global__ __launch_bounds__(kWinBlock) __attribute__((amdgpu_waves_per_eu(3))) 
     // flight while the window is staged, so a slice up to 2 kD blocks wide
     // waits on memory once.  (At 3 waves per SIMD, which the windows' LDS
     // leaves, loads two steps ahead left the steps waiting on memory: 111
-    // against 103 us without windows at k = 16.)  The window is staged kR
-    // pieces per lane per round, all in flight together (one round for up to
-    // 96 tiles).
+    // against 103 us without windows at k = 16.)  The loads of a staging
+    // round are all in flight together: a round takes up to 96 tiles of tiled
+    // X (kR 16-byte pieces per lane) or 64 of column-major X (one tile per
+    // lane).
     constexpr int kD = 6, PP = 2 * NQ, kR = 3 * NQ, KC = 4 * NQ;
     extern __shared__ __align__(16) unsigned char spmm_lds[];
     const int lane = threadIdx.x & (kWave - 1);
