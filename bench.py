@@ -288,8 +288,16 @@ def run_spmm(args):
         dX = cs.DeviceBuffer(ctx, Xb)
         dY = cs.DeviceVector(ctx, k * ldy, np.zeros(k * ldy, np.float32))
         tiled = args.x_layout == "tiled"
+        pack_ms = None
         if tiled:  # X handed over in 4-row tiles: packed once, outside the timed region
             dX4 = cs.spmm_pack_x(ctx, k, A.width, dX, ldx)
+            # what packing a column-major X costs (host-timed, 20 calls)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(20):
+                cs.check(cs.lib.clspmv_spmm_pack_x(ctx.h, k, A.width, dX.p, ldx, dX4.p), "spmm_pack_x")
+            ctx.sync()
+            pack_ms = (time.perf_counter() - t0) * 1e3 / 20
 
         def product(accumulate=False):
             if tiled:
@@ -362,6 +370,7 @@ def run_spmm(args):
                 "traffic": measured_traffic(f"spmm-k{k}", "bspmm4") if args.dtype == "bf16" else None},
             "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 3),
             "column_major_x_ms": round(other_ms, 6) if other_ms is not None else None,
+            "pack_x_ms": round(pack_ms, 6) if pack_ms is not None else None,
             "error_vs_fp64_stored_values": [float(f"{e:.3e}") for e in errs],
             "error_vs_fp64_fp32_inputs": [float(f"{e:.3e}") for e in qerrs],
             "matrix_error_vs_fp64": {kk: (float(f"{v:.3e}") if isinstance(v, float) else v)
