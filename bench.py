@@ -257,9 +257,10 @@ def spmm_bytes(blocks, brows, rows, cols, k, s_v=2, s_x=2, y_read=True):
     return blocks * 16 * s_v + 4 * blocks + 4 * (brows + 1) + s_x * cols * k + (2 if y_read else 1) * 4 * rows * k
 
 
-def run_spmm(args):
+def run_spmm(args, collect=None):
     """Mixed-precision MFMA block SpMM on the 4x4-block matrix (524,288 block
-    rows x 12 blocks): one JSON line per k."""
+    rows x 12 blocks): one JSON line per k (appended to `collect` instead of
+    printed when given)."""
     import numpy as np
     import clspmv_mic_amd as cs
     from clspmv_mic_amd import matrices
@@ -377,7 +378,10 @@ def run_spmm(args):
                                      for kk, v in matrix_error.items()},
             "cpu_baseline": None,
         }
-        print(json.dumps(out), flush=True)
+        if collect is None:
+            print(json.dumps(out), flush=True)
+        else:
+            collect.append(out)
     return 0
 
 
@@ -716,6 +720,7 @@ def main():
                     help="format: block height and width")
     ap.add_argument("--tables", default=None, help="directory holding benchmark/*.ben")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-spmm", action="store_true", help="headline: skip the block SpMM summary")
     ap.add_argument("--no-tune", action="store_true", help="keep the tables' methods and graph launches")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -756,6 +761,18 @@ def main():
     if len(results) > 1:  # the power-law matrix's numbers, measured the same way
         extra = {k: v for k, v in results[1].items() if k != "metric"}
         out["powerlaw"] = extra
+    if args.workload == "headline" and not args.no_spmm:
+        # The BF16 block SpMM (Y = A X, X in 4-row tiles) at k = 8 / 16 / 32, in
+        # brief (bench.py --workload spmm prints the full lines).
+        sp_args = argparse.Namespace(**{**vars(args), "k": [8, 16, 32], "err_columns": 1, "dtype": "bf16",
+                                        "no_accumulate": False, "x_layout": "tiled"})
+        lines = []
+        run_spmm(sp_args, collect=lines)
+        out["spmm"] = [{"k": d["config"]["k"], "ms_y_eq_ax": d["ms_per_step"], "tflops": d["mfma_tflops"],
+                        "frac": d["roofline"]["frac"], "traffic": d["roofline"]["traffic"],
+                        "ms_y_plus_ax": (d["y_accumulate"] or {}).get("ms"),
+                        "frac_y_plus_ax": (d["y_accumulate"] or {}).get("frac"),
+                        "max_error_vs_fp64": max(d["error_vs_fp64_stored_values"])} for d in lines]
     print(json.dumps(out))
     return 0
 
