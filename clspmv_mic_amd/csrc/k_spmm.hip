@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kWinBlock) __attribute__((amdgpu_waves_per_eu(3))) 
     // leaves, loads two steps ahead left the steps waiting on memory: 111
     // against 103 us without windows at k = 16.)  The loads of a staging
     // round are all in flight together: a round takes up to 96 tiles of tiled
-    // X (kR 16-byte pieces per lane) or 64 of column-major X (one tile per
+    // X (kR 16-byte pieces per lane) or 128 of column-major X (two tiles per
     // lane).
     constexpr int kD = 6, PP = 2 * NQ, kR = 3 * NQ, KC = 4 * NQ;
     extern __shared__ __align__(16) unsigned char spmm_lds[];
@@ -409,26 +409,40 @@ __global__ __launch_bounds__(kWinBlock) __attribute__((amdgpu_waves_per_eu(3))) 
     auto st_colx = [&](int cc) {
         const char* const x_base = reinterpret_cast<const char*>(Xall + (size_t)cc * ldx);
         const int kc = MULTI ? min(KC, kall - cc) : kall;
-        for (int j0 = 0; j0 < win.max; j0 += kWave) {
-            const int j = j0 + lane, jc = min(j, win.max - 1);
-            const int tl = (MULTI && cc > 0) ? wlist_lds[jc] : w_list[jc];
-            if (MULTI && cc == 0 && j < win.max) wlist_lds[j] = tl;
-            s16x4 v[KC];
+        constexpr int kT = 2;  // tiles per lane per round: one round for up to 128 tiles
+        for (int j0 = 0; j0 < win.max; j0 += kT * kWave) {
+            int tl[kT];
 #pragma unroll
-            for (int r = 0; r < KC; ++r)
-                v[r] = *reinterpret_cast<const s16x4*>(x_base +
-                                                      ((r < kc ? (unsigned)r : 0u) * (unsigned)ldx + (unsigned)tl * 4) * 2);
-            if (cols & 3) {  // wave-uniform: zeros past the last column
-                const int past = tl * 4 - cols;  // rows 4 tl + e with e >= -past are past
+            for (int h = 0; h < kT; ++h) {
+                const int j = j0 + h * kWave + lane, jc = min(j, win.max - 1);
+                tl[h] = (MULTI && cc > 0) ? wlist_lds[jc] : w_list[jc];
+                if (MULTI && cc == 0 && j < win.max) wlist_lds[j] = tl[h];
+            }
+            s16x4 v[kT][KC];
+#pragma unroll
+            for (int h = 0; h < kT; ++h)
 #pragma unroll
                 for (int r = 0; r < KC; ++r)
+                    v[h][r] = *reinterpret_cast<const s16x4*>(
+                        x_base + ((r < kc ? (unsigned)r : 0u) * (unsigned)ldx + (unsigned)tl[h] * 4) * 2);
+            if (cols & 3) {  // wave-uniform: zeros past the last column
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (e + past >= 0) v[r][e] = 0;
+                for (int h = 0; h < kT; ++h) {
+                    const int past = tl[h] * 4 - cols;  // rows 4 tl + e with e >= -past are past
+#pragma unroll
+                    for (int r = 0; r < KC; ++r)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (e + past >= 0) v[h][r][e] = 0;
+                }
             }
-            unsigned char* const dst = wbuf + (j < win.max ? j : spare) * kWinTile<NQ>;
 #pragma unroll
-            for (int r = 0; r < KC; ++r) *reinterpret_cast<s16x4*>(dst + r * 8) = v[r];
+            for (int h = 0; h < kT; ++h) {
+                const int j = j0 + h * kWave + lane;
+                unsigned char* const dst = wbuf + (j < win.max ? j : spare) * kWinTile<NQ>;
+#pragma unroll
+                for (int r = 0; r < KC; ++r) *reinterpret_cast<s16x4*>(dst + r * 8) = v[h][r];
+            }
         }
     };
     // The wave reads only what its own lanes wrote and its LDS accesses
