@@ -724,8 +724,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the tables' methods and graph launches")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); nothing is measured on more "
-              "GPUs in this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the engine runs one device "
+              "per process and has no multi-GPU path to measure.", file=sys.stderr)
         return 2
 
     if args.workload == "spmm":
